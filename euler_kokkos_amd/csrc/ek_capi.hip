@@ -1,0 +1,464 @@
+// ek_capi.hip -- the C ABI of include/ek_hip.h: device state, uploads/downloads, the
+// device-resident time loop (captured into a HIP graph) and the budget reductions.
+#include "ek_hip.h"
+#include "ek_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+struct ek_ctx {
+  ek_params p{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  void* U[2] = { nullptr, nullptr };
+  int cur = 0;
+  ek::TimeSlot* ts = nullptr;
+  unsigned long long* scratch_bits = nullptr;
+  double* partial = nullptr;
+  double* stage = nullptr;   // FP64 staging buffer for reference-layout transfers
+  long long stage_elems = 0;
+  int nblocks_budget = 0;
+  size_t elem = 8;
+  long long ncell = 0;
+  ek::Geom<double> g64{};
+  ek::Gas<double> gas64{};
+  ek::Geom<float> g32{};
+  ek::Gas<float> gas32{};
+  int st = 2;
+  int cz = 32;
+  long long bytes = 0;
+  // captured 6-step graph (6 = lcm of the 2 state buffers and the 3 time slots)
+  hipGraphExec_t graph = nullptr;
+  int graph_nStepmax = -1;
+  double graph_tEnd = 0.0;
+  void* graph_U0 = nullptr;
+};
+
+static std::string g_create_err;
+
+static int fail(ek_ctx* c, int code, const std::string& msg)
+{
+  if (c)
+    c->err = msg;
+  else
+    g_create_err = msg;
+  return code;
+}
+
+static int hip_fail(ek_ctx* c, hipError_t e, const char* what)
+{
+  return fail(c, EK_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define EK_HIP(c, call)                      \
+  do {                                       \
+    hipError_t e_ = (call);                  \
+    if (e_ != hipSuccess)                    \
+      return hip_fail((c), e_, #call);       \
+  } while (0)
+
+static bool is_pow2(double x)
+{
+  if (!(x > 0.0) || !std::isfinite(x))
+    return false;
+  int e;
+  const double m = std::frexp(x, &e);
+  return m == 0.5;
+}
+
+template <class T>
+static void fill_geom(const ek_params& p, ek::Geom<T>& g, ek::Gas<T>& gas)
+{
+  g.nx = p.nx;
+  g.ny = p.ny;
+  g.nz = p.dim == 3 ? p.nz : 1;
+  for (int i = 0; i < 6; i++)
+    g.bc[i] = p.bc[i];
+  g.plane = (long long)p.nx * p.ny;
+  g.vstride = g.plane * g.nz;
+  g.dx = p.dx;
+  g.dy = p.dy;
+  g.dz = p.dz;
+  g.tdx = (T)p.dx;
+  g.tdy = (T)p.dy;
+  g.tdz = (T)p.dz;
+  g.pow2 = is_pow2(p.dx) && is_pow2(p.dy) && (p.dim == 2 || is_pow2(p.dz));
+  g.idx = (T)(1.0 / p.dx);
+  g.idy = (T)(1.0 / p.dy);
+  g.idz = (T)(1.0 / p.dz);
+  gas.gamma0 = (T)p.gamma0;
+  gas.gamma6 = (T)p.gamma6;
+  gas.gm1 = (T)(p.gamma0 - 1.0);
+  gas.smallr = (T)p.smallr;
+  gas.smallc = (T)p.smallc;
+  gas.smallp = (T)p.smallp;
+  gas.smallpp = (T)p.smallpp;
+  gas.smallc2 = (T)(p.smallc * p.smallc);
+  gas.entho = (T)(1.0 / (p.gamma0 - 1.0));
+}
+
+static int validate(const ek_params* p, std::string& why)
+{
+  if (p->dim != 2 && p->dim != 3)
+    return why = "dim must be 2 or 3", 1;
+  if (p->mhd)
+    return why = "MHD solvers are not provided by this library build", 1;
+  if (p->gw != 2)
+    return why = "hydro MUSCL needs ghostWidth 2", 1;
+  if (p->nbvar != (p->dim == 2 ? 4 : 5))
+    return why = "nbvar must be 4 (2D) or 5 (3D) for hydro", 1;
+  if (p->nx < p->gw || p->ny < p->gw || (p->dim == 3 && p->nz < p->gw))
+    return why = "every axis needs at least ghostWidth interior cells", 1;
+  if (p->precision != EK_FP64 && p->precision != EK_FP32)
+    return why = "precision must be EK_FP64 or EK_FP32", 1;
+  if (p->riemann != ek::RS_APPROX && p->riemann != ek::RS_LLF && p->riemann != ek::RS_HLLC &&
+      !(p->riemann == ek::RS_HLL && p->dim == 2))
+    return why = "riemann solver not available for hydro in this dimension (3D has no HLL in the "
+                 "reference dispatch; HLLD is MHD-only)",
+           1;
+  if (!(p->slope_type == 0.0 || p->slope_type == 1.0 || p->slope_type == 2.0))
+    return why = "slope_type must be 0, 1 or 2", 1;
+  for (int i = 0; i < 2 * p->dim; i++)
+    if (p->bc[i] < 1 || p->bc[i] > 3)
+      return why = "boundary types must be 1 (reflecting), 2 (outflow) or 3 (periodic)", 1;
+  if (!(p->dx > 0) || !(p->dy > 0) || (p->dim == 3 && !(p->dz > 0)))
+    return why = "cell sizes must be positive", 1;
+  return 0;
+}
+
+extern "C" int ek_create(const ek_params* params, ek_ctx** out)
+{
+  if (!params || !out)
+    return fail(nullptr, EK_EINVAL, "null argument");
+  *out = nullptr;
+  std::string why;
+  if (validate(params, why))
+    return fail(nullptr, EK_EINVAL, why);
+  ek_ctx* c = new ek_ctx();
+  c->p = *params;
+  c->st = (int)params->slope_type;
+  c->elem = params->precision == EK_FP64 ? 8 : 4;
+  fill_geom<double>(c->p, c->g64, c->gas64);
+  fill_geom<float>(c->p, c->g32, c->gas32);
+  c->ncell = c->g64.vstride;
+  c->cz = params->dim == 3 ? (params->nz < 32 ? params->nz : 32) : 1;
+  hipError_t e;
+  if ((e = hipGetDevice(&c->device)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+    std::string m = std::string("HIP init: ") + hipGetErrorString(e);
+    delete c;
+    return fail(nullptr, EK_EHIP, m);
+  }
+  const size_t state_bytes = (size_t)c->ncell * c->p.nbvar * c->elem;
+  c->nblocks_budget = (int)std::min<long long>(1024, (c->ncell + 255) / 256);
+  if (hipMalloc(&c->U[0], state_bytes) != hipSuccess || hipMalloc(&c->U[1], state_bytes) != hipSuccess ||
+      hipMalloc(&c->ts, 4 * sizeof(ek::TimeSlot)) != hipSuccess ||
+      hipMalloc(&c->scratch_bits, 64) != hipSuccess ||
+      hipMalloc(&c->partial, (size_t)c->nblocks_budget * 16 * sizeof(double)) != hipSuccess) {
+    ek_destroy(c);
+    return fail(nullptr, EK_ENOMEM, "device allocation failed");
+  }
+  c->bytes = (long long)(2 * state_bytes + 4 * sizeof(ek::TimeSlot) + 64 +
+                         (size_t)c->nblocks_budget * 16 * sizeof(double));
+  hipMemsetAsync(c->U[0], 0, state_bytes, c->stream);
+  hipMemsetAsync(c->U[1], 0, state_bytes, c->stream);
+  hipMemsetAsync(c->ts, 0, 4 * sizeof(ek::TimeSlot), c->stream);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+    ek_destroy(c);
+    return fail(nullptr, EK_EHIP, std::string("HIP init: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return EK_OK;
+}
+
+extern "C" void ek_destroy(ek_ctx* c)
+{
+  if (!c)
+    return;
+  if (c->stream)
+    hipStreamSynchronize(c->stream);
+  if (c->graph)
+    hipGraphExecDestroy(c->graph);
+  hipFree(c->U[0]);
+  hipFree(c->U[1]);
+  hipFree(c->ts);
+  hipFree(c->scratch_bits);
+  hipFree(c->partial);
+  hipFree(c->stage);
+  if (c->stream)
+    hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" const char* ek_last_error(const ek_ctx* c)
+{
+  return c ? c->err.c_str() : g_create_err.c_str();
+}
+
+static long long ref_elems(const ek_params& p)
+{
+  const long long is = p.nx + 2LL * p.gw, js = p.ny + 2LL * p.gw, ks = p.nz + 2LL * p.gw;
+  return is * js * (p.dim == 3 ? ks : 1) * p.nbvar;
+}
+
+static int ensure_stage(ek_ctx* c, long long n)
+{
+  if (c->stage_elems >= n)
+    return EK_OK;
+  hipFree(c->stage);
+  c->stage = nullptr;
+  c->stage_elems = 0;
+  if (hipMalloc(&c->stage, (size_t)n * sizeof(double)) != hipSuccess)
+    return fail(c, EK_ENOMEM, "staging allocation failed");
+  c->stage_elems = n;
+  return EK_OK;
+}
+
+extern "C" int ek_upload(ek_ctx* c, const double* host, int layout)
+{
+  if (!c || !host)
+    return fail(c, EK_EINVAL, "null argument");
+  const ek_params& p = c->p;
+  const int is = p.nx + 2 * p.gw, js = p.ny + 2 * p.gw, ks = p.nz + 2 * p.gw;
+  if (layout == EK_LAYOUT_SOA) {
+    const long long n = c->ncell * p.nbvar;
+    if (p.precision == EK_FP64) {
+      EK_HIP(c, hipMemcpyAsync(c->U[c->cur], host, n * 8, hipMemcpyHostToDevice, c->stream));
+    } else {
+      std::vector<float> tmp(n);
+      for (long long i = 0; i < n; i++)
+        tmp[i] = (float)host[i];
+      EK_HIP(c, hipMemcpyAsync(c->U[c->cur], tmp.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+      EK_HIP(c, hipStreamSynchronize(c->stream));
+    }
+  } else if (layout == EK_LAYOUT_REFERENCE) {
+    const long long n = ref_elems(p);
+    if (int r = ensure_stage(c, n))
+      return r;
+    EK_HIP(c, hipMemcpyAsync(c->stage, host, n * 8, hipMemcpyHostToDevice, c->stream));
+    if (p.precision == EK_FP64)
+      EK_HIP(c, ek::fp64::ref_to_soa(p.dim, c->stage, (double*)c->U[c->cur], c->g64, p.gw, p.nbvar, is, js,
+                                    ks, c->stream));
+    else
+      EK_HIP(c, ek::fp32::ref_to_soa(p.dim, c->stage, (float*)c->U[c->cur], c->g32, p.gw, p.nbvar, is, js,
+                                    ks, c->stream));
+  } else {
+    return fail(c, EK_EINVAL, "unknown layout");
+  }
+  EK_HIP(c, hipStreamSynchronize(c->stream));
+  return EK_OK;
+}
+
+extern "C" int ek_download(ek_ctx* c, double* host, int layout)
+{
+  if (!c || !host)
+    return fail(c, EK_EINVAL, "null argument");
+  const ek_params& p = c->p;
+  const int is = p.nx + 2 * p.gw, js = p.ny + 2 * p.gw, ks = p.nz + 2 * p.gw;
+  if (layout == EK_LAYOUT_SOA) {
+    const long long n = c->ncell * p.nbvar;
+    if (p.precision == EK_FP64) {
+      EK_HIP(c, hipMemcpyAsync(host, c->U[c->cur], n * 8, hipMemcpyDeviceToHost, c->stream));
+      EK_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+      std::vector<float> tmp(n);
+      EK_HIP(c, hipMemcpyAsync(tmp.data(), c->U[c->cur], n * 4, hipMemcpyDeviceToHost, c->stream));
+      EK_HIP(c, hipStreamSynchronize(c->stream));
+      for (long long i = 0; i < n; i++)
+        host[i] = (double)tmp[i];
+    }
+  } else if (layout == EK_LAYOUT_REFERENCE) {
+    const long long n = ref_elems(p);
+    if (int r = ensure_stage(c, n))
+      return r;
+    if (p.precision == EK_FP64)
+      EK_HIP(c, ek::fp64::soa_to_ref(p.dim, (const double*)c->U[c->cur], c->stage, c->g64, p.gw, p.nbvar, is,
+                                    js, ks, p.mhd, c->stream));
+    else
+      EK_HIP(c, ek::fp32::soa_to_ref(p.dim, (const float*)c->U[c->cur], c->stage, c->g32, p.gw, p.nbvar, is,
+                                    js, ks, p.mhd, c->stream));
+    EK_HIP(c, hipMemcpyAsync(host, c->stage, n * 8, hipMemcpyDeviceToHost, c->stream));
+    EK_HIP(c, hipStreamSynchronize(c->stream));
+  } else {
+    return fail(c, EK_EINVAL, "unknown layout");
+  }
+  return EK_OK;
+}
+
+static hipError_t launch_inv_dt_into(ek_ctx* c, unsigned long long* dst)
+{
+  if (c->p.precision == EK_FP64)
+    return ek::fp64::inv_dt(c->p.dim, (const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
+  return ek::fp32::inv_dt(c->p.dim, (const float*)c->U[c->cur], c->g32, c->gas32, dst, c->stream);
+}
+
+extern "C" int ek_inv_dt(ek_ctx* c, double* inv_dt)
+{
+  if (!c || !inv_dt)
+    return fail(c, EK_EINVAL, "null argument");
+  EK_HIP(c, hipMemsetAsync(c->scratch_bits, 0, 8, c->stream));
+  EK_HIP(c, launch_inv_dt_into(c, c->scratch_bits));
+  unsigned long long b = 0;
+  EK_HIP(c, hipMemcpyAsync(&b, c->scratch_bits, 8, hipMemcpyDeviceToHost, c->stream));
+  EK_HIP(c, hipStreamSynchronize(c->stream));
+  std::memcpy(inv_dt, &b, 8);
+  return EK_OK;
+}
+
+static hipError_t launch_one(ek_ctx* c, const ek::StepCtl& ctl, void* in, void* out)
+{
+  const ek_params& p = c->p;
+  if (p.precision == EK_FP64)
+    return ek::fp64::step(p.dim, p.riemann, c->st, (const double*)in, (double*)out, c->g64, c->gas64, ctl,
+                          c->cz, c->stream);
+  return ek::fp32::step(p.dim, p.riemann, c->st, (const float*)in, (float*)out, c->g32, c->gas32, ctl, c->cz,
+                        c->stream);
+}
+
+extern "C" int ek_step(ek_ctx* c, double dt)
+{
+  if (!c)
+    return fail(c, EK_EINVAL, "null argument");
+  EK_HIP(c, hipMemsetAsync(c->ts, 0, 3 * sizeof(ek::TimeSlot), c->stream));
+  ek::StepCtl ctl{};
+  ctl.ts = c->ts;
+  ctl.slot = 0;
+  ctl.mode = 1;
+  ctl.dt_given = dt;
+  ctl.cfl = c->p.cfl;
+  ctl.tEnd = 0.0;
+  ctl.nStepmax = 0;
+  EK_HIP(c, launch_one(c, ctl, c->U[c->cur], c->U[1 - c->cur]));
+  c->cur = 1 - c->cur;
+  return EK_OK;
+}
+
+static int build_graph(ek_ctx* c, int nStepmax, double tEnd)
+{
+  if (c->graph) {
+    hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  EK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  for (int i = 0; i < 6; i++) {
+    ek::StepCtl ctl{};
+    ctl.ts = c->ts;
+    ctl.slot = i % 3;
+    ctl.mode = 0;
+    ctl.cfl = c->p.cfl;
+    ctl.tEnd = tEnd;
+    ctl.nStepmax = nStepmax;
+    hipError_t e = launch_one(c, ctl, c->U[i % 2], c->U[(i + 1) % 2]);
+    if (e != hipSuccess) {
+      hipStreamEndCapture(c->stream, &g);
+      if (g)
+        hipGraphDestroy(g);
+      return hip_fail(c, e, "graph capture launch");
+    }
+  }
+  EK_HIP(c, hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return hip_fail(c, e, "hipGraphInstantiate");
+  c->graph_nStepmax = nStepmax;
+  c->graph_tEnd = tEnd;
+  c->graph_U0 = c->U[0];
+  return EK_OK;
+}
+
+extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* t_io, int* it_io,
+                      double* dt_last)
+{
+  if (!c || !t_io || !it_io)
+    return fail(c, EK_EINVAL, "null argument");
+  if (nsteps < 0)
+    return fail(c, EK_EINVAL, "nsteps must be >= 0");
+  // the current state always starts in U[0] so that the captured graph's buffer order holds
+  if (c->cur == 1) {
+    std::swap(c->U[0], c->U[1]);
+    c->cur = 0;
+  }
+  ek::TimeSlot init[3];
+  std::memset(init, 0, sizeof(init));
+  init[0].t = *t_io;
+  init[0].it = *it_io;
+  EK_HIP(c, hipMemcpyAsync(c->ts, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  EK_HIP(c, launch_inv_dt_into(c, &c->ts[0].invdt_bits));
+  const int nfull = nsteps / 6, rem = nsteps % 6;
+  if (nfull > 0) {
+    if (!c->graph || c->graph_nStepmax != nStepmax || c->graph_tEnd != tEnd || c->graph_U0 != c->U[0])
+      if (int r = build_graph(c, nStepmax, tEnd))
+        return r;
+    for (int g = 0; g < nfull; g++)
+      EK_HIP(c, hipGraphLaunch(c->graph, c->stream));
+  }
+  for (int r = 0; r < rem; r++) {
+    ek::StepCtl ctl{};
+    ctl.ts = c->ts;
+    ctl.slot = r % 3;
+    ctl.mode = 0;
+    ctl.cfl = c->p.cfl;
+    ctl.tEnd = tEnd;
+    ctl.nStepmax = nStepmax;
+    EK_HIP(c, launch_one(c, ctl, c->U[r % 2], c->U[(r + 1) % 2]));
+  }
+  ek::TimeSlot fin;
+  EK_HIP(c, hipMemcpyAsync(&fin, &c->ts[nsteps % 3], sizeof(fin), hipMemcpyDeviceToHost, c->stream));
+  EK_HIP(c, hipStreamSynchronize(c->stream));
+  const int ran = fin.it - *it_io;
+  c->cur = ran % 2;
+  *t_io = fin.t;
+  *it_io = fin.it;
+  if (dt_last)
+    *dt_last = fin.dt;
+  return EK_OK;
+}
+
+extern "C" int ek_budgets(ek_ctx* c, double* sums)
+{
+  if (!c || !sums)
+    return fail(c, EK_EINVAL, "null argument");
+  const int nv = c->p.nbvar;
+  const int nb = c->nblocks_budget;
+  if (c->p.precision == EK_FP64)
+    EK_HIP(c, ek::fp64::budget(nv, (const double*)c->U[c->cur], c->ncell, c->ncell, c->partial, nb, c->stream));
+  else
+    EK_HIP(c, ek::fp32::budget(nv, (const float*)c->U[c->cur], c->ncell, c->ncell, c->partial, nb, c->stream));
+  std::vector<double> h((size_t)nb * nv * 2);
+  EK_HIP(c, hipMemcpyAsync(h.data(), c->partial, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  EK_HIP(c, hipStreamSynchronize(c->stream));
+  for (int v = 0; v < nv; v++) {
+    double S = 0.0, C = 0.0;
+    for (int b = 0; b < nb; b++)
+      for (int k = 0; k < 2; k++) {
+        const double x = h[((size_t)b * nv + v) * 2 + k];
+        const double t = S + x;
+        if (std::fabs(S) >= std::fabs(x))
+          C += (S - t) + x;
+        else
+          C += (x - t) + S;
+        S = t;
+      }
+    sums[v] = S + C;
+  }
+  return EK_OK;
+}
+
+extern "C" int ek_sync(ek_ctx* c)
+{
+  if (!c)
+    return fail(c, EK_EINVAL, "null argument");
+  EK_HIP(c, hipStreamSynchronize(c->stream));
+  return EK_OK;
+}
+
+extern "C" long long ek_device_bytes(const ek_ctx* c)
+{
+  return c ? c->bytes + c->stage_elems * 8 : 0;
+}

@@ -1,0 +1,32 @@
+// ek_internal.h -- per-precision launch entry points, implemented in ek_fp64.hip
+// (bit-exact FP64) and ek_fp32.hip (FP32 bulk), called by the C ABI in ek_capi.hip.
+#pragma once
+
+#include "ek_hydro.h"
+
+namespace ek {
+namespace fp64 {
+hipError_t step(int dim, int rs, int st, const double* Uin, double* Uout, const Geom<double>& geo,
+                const Gas<double>& gas, const StepCtl& ctl, int cz, hipStream_t s);
+hipError_t inv_dt(int dim, const double* U, const Geom<double>& geo, const Gas<double>& gas,
+                  unsigned long long* dst, hipStream_t s);
+hipError_t budget(int nv, const double* U, long long ncell, long long vstride, double* partial,
+                  int nblocks, hipStream_t s);
+hipError_t ref_to_soa(int dim, const double* R, double* U, const Geom<double>& geo, int gw, int nbvar,
+                      int isize, int jsize, int ksize, hipStream_t s);
+hipError_t soa_to_ref(int dim, const double* U, double* R, const Geom<double>& geo, int gw, int nbvar,
+                      int isize, int jsize, int ksize, int mhd, hipStream_t s);
+} // namespace fp64
+namespace fp32 {
+hipError_t step(int dim, int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo,
+                const Gas<float>& gas, const StepCtl& ctl, int cz, hipStream_t s);
+hipError_t inv_dt(int dim, const float* U, const Geom<float>& geo, const Gas<float>& gas,
+                  unsigned long long* dst, hipStream_t s);
+hipError_t budget(int nv, const float* U, long long ncell, long long vstride, double* partial,
+                  int nblocks, hipStream_t s);
+hipError_t ref_to_soa(int dim, const double* R, float* U, const Geom<float>& geo, int gw, int nbvar,
+                      int isize, int jsize, int ksize, hipStream_t s);
+hipError_t soa_to_ref(int dim, const float* U, double* R, const Geom<float>& geo, int gw, int nbvar,
+                      int isize, int jsize, int ksize, int mhd, hipStream_t s);
+} // namespace fp32
+} // namespace ek

@@ -1,0 +1,163 @@
+"""ctypes binding of the C ABI in ``include/ek_hip.h`` (``libek_hip.so``).
+
+The library is loaded from the package directory (built in-tree by
+``python -m euler_kokkos_amd.build``). There is no Python or CPU fallback: if the
+library is missing, :func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libek_hip.so")
+
+EK_FP64, EK_FP32 = 0, 1
+EK_LAYOUT_REFERENCE, EK_LAYOUT_SOA = 0, 1
+RIEMANN = {"approx": 0, "llf": 1, "hll": 2, "hllc": 3, "hlld": 4}
+
+# Exported symbols declared by include/ek_hip.h (checked by the CPU test-suite).
+EXPORTS = ("ek_create", "ek_destroy", "ek_last_error", "ek_upload", "ek_download", "ek_inv_dt",
+           "ek_step", "ek_run", "ek_budgets", "ek_sync", "ek_device_bytes")
+
+
+class EkParams(C.Structure):
+    """Mirror of ``ek_params`` (include/ek_hip.h)."""
+
+    _fields_ = [
+        ("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("gw", C.c_int), ("nbvar", C.c_int), ("mhd", C.c_int), ("riemann", C.c_int),
+        ("bc", C.c_int * 6), ("precision", C.c_int), ("reserved", C.c_int),
+        ("xmin", C.c_double), ("ymin", C.c_double), ("zmin", C.c_double),
+        ("xmax", C.c_double), ("ymax", C.c_double), ("zmax", C.c_double),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("gamma0", C.c_double), ("gamma6", C.c_double), ("cfl", C.c_double),
+        ("slope_type", C.c_double), ("smallr", C.c_double), ("smallc", C.c_double),
+        ("smallp", C.c_double), ("smallpp", C.c_double), ("cIso", C.c_double),
+    ]
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Load libek_hip.so once; raise if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run `python -m euler_kokkos_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.ek_create.argtypes = [P(EkParams), P(C.c_void_p)]
+    L.ek_create.restype = C.c_int
+    L.ek_destroy.argtypes = [C.c_void_p]
+    L.ek_destroy.restype = None
+    L.ek_last_error.argtypes = [C.c_void_p]
+    L.ek_last_error.restype = C.c_char_p
+    L.ek_upload.argtypes = [C.c_void_p, P(C.c_double), C.c_int]
+    L.ek_download.argtypes = [C.c_void_p, P(C.c_double), C.c_int]
+    L.ek_inv_dt.argtypes = [C.c_void_p, P(C.c_double)]
+    L.ek_step.argtypes = [C.c_void_p, C.c_double]
+    L.ek_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, P(C.c_double), P(C.c_int),
+                         P(C.c_double)]
+    L.ek_budgets.argtypes = [C.c_void_p, P(C.c_double)]
+    L.ek_sync.argtypes = [C.c_void_p]
+    L.ek_device_bytes.argtypes = [C.c_void_p]
+    L.ek_device_bytes.restype = C.c_longlong
+    for n in ("ek_upload", "ek_download", "ek_inv_dt", "ek_step", "ek_run", "ek_budgets", "ek_sync"):
+        getattr(L, n).restype = C.c_int
+    _LIB = L
+    return L
+
+
+class EkError(RuntimeError):
+    pass
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One device state (``ek_ctx``): a hydro grid resident in HBM."""
+
+    def __init__(self, params: EkParams):
+        self._L = lib()
+        h = C.c_void_p()
+        rc = self._L.ek_create(C.byref(params), C.byref(h))
+        if rc != 0:
+            raise EkError(f"ek_create failed ({rc}): {self._L.ek_last_error(None).decode()}")
+        self.h = h
+        self.params = params
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise EkError(f"{what} failed ({rc}): {self._L.ek_last_error(self.h).decode()}")
+
+    def close(self) -> None:
+        if self.h:
+            self._L.ek_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # shapes --------------------------------------------------------------------------
+    def ref_shape(self):
+        p = self.params
+        s = [p.nx + 2 * p.gw, p.ny + 2 * p.gw]
+        if p.dim == 3:
+            s.append(p.nz + 2 * p.gw)
+        return tuple(s) + (p.nbvar,)
+
+    def soa_shape(self):
+        p = self.params
+        if p.dim == 2:
+            return (p.nbvar, p.ny, p.nx)
+        return (p.nbvar, p.nz, p.ny, p.nx)
+
+    # transfers -----------------------------------------------------------------------
+    def upload(self, arr: np.ndarray, layout: int = EK_LAYOUT_REFERENCE) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        want = self.ref_shape() if layout == EK_LAYOUT_REFERENCE else self.soa_shape()
+        if a.shape != want:
+            raise ValueError(f"shape {a.shape} != {want}")
+        self._check(self._L.ek_upload(self.h, _dptr(a), layout), "ek_upload")
+
+    def download(self, layout: int = EK_LAYOUT_REFERENCE) -> np.ndarray:
+        shape = self.ref_shape() if layout == EK_LAYOUT_REFERENCE else self.soa_shape()
+        a = np.empty(shape, dtype=np.float64)
+        self._check(self._L.ek_download(self.h, _dptr(a), layout), "ek_download")
+        return a
+
+    # compute -------------------------------------------------------------------------
+    def inv_dt(self) -> float:
+        v = C.c_double()
+        self._check(self._L.ek_inv_dt(self.h, C.byref(v)), "ek_inv_dt")
+        return v.value
+
+    def step(self, dt: float) -> None:
+        self._check(self._L.ek_step(self.h, dt), "ek_step")
+
+    def run(self, nsteps: int, nStepmax: int, tEnd: float, t: float = 0.0, it: int = 0):
+        tt, ii, dd = C.c_double(t), C.c_int(it), C.c_double(0.0)
+        self._check(self._L.ek_run(self.h, nsteps, nStepmax, tEnd, C.byref(tt), C.byref(ii),
+                                   C.byref(dd)), "ek_run")
+        return tt.value, ii.value, dd.value
+
+    def budgets(self) -> np.ndarray:
+        out = np.zeros(8, dtype=np.float64)
+        self._check(self._L.ek_budgets(self.h, _dptr(out)), "ek_budgets")
+        return out[: self.params.nbvar]
+
+    def sync(self) -> None:
+        self._check(self._L.ek_sync(self.h), "ek_sync")
+
+    def device_bytes(self) -> int:
+        return int(self._L.ek_device_bytes(self.h))

@@ -1,0 +1,138 @@
+"""ctypes binding of the oracle (oracle/libek_oracle.so).
+
+TEST INFRASTRUCTURE: the CPU restatement of the reference step that the HIP path is
+checked against. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libek_oracle.so")
+
+
+class EkoParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("gw", C.c_int),
+        ("nbvar", C.c_int), ("isize", C.c_int), ("jsize", C.c_int), ("ksize", C.c_int),
+        ("imin", C.c_int), ("imax", C.c_int), ("jmin", C.c_int), ("jmax", C.c_int),
+        ("kmin", C.c_int), ("kmax", C.c_int), ("bc", C.c_int * 6), ("riemann", C.c_int),
+        ("mhd", C.c_int),
+        ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
+        ("zmin", C.c_double), ("zmax", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+        ("dz", C.c_double), ("gamma0", C.c_double), ("gamma6", C.c_double), ("cfl", C.c_double),
+        ("slope_type", C.c_double), ("smallr", C.c_double), ("smallc", C.c_double),
+        ("smallp", C.c_double), ("smallpp", C.c_double), ("cIso", C.c_double),
+    ]
+
+
+class EkoImplode(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rho_out", "p_out", "u_out", "v_out", "w_out",
+                                          "rho_in", "p_in", "u_in", "v_in", "w_in")] + \
+               [("shape", C.c_int)]
+
+
+class EkoBlast(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("radius", "center_x", "center_y", "center_z",
+                                          "density_in", "density_out", "pressure_in",
+                                          "pressure_out")]
+
+
+_L = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        D = P(C.c_double)
+        L.eko_params_init.argtypes = [P(EkoParams)]
+        L.eko_compute_primitives.argtypes = [P(EkoParams), D, D, D]
+        L.eko_slope_scalar.argtypes = [P(EkoParams), C.c_double, C.c_double, C.c_double, D]
+        L.eko_trace_along_dir.argtypes = [P(EkoParams), D, D, D, D, C.c_double, C.c_double,
+                                          C.c_double, C.c_int, D]
+        L.eko_riemann_hydro.argtypes = [P(EkoParams), D, D, D]
+        L.eko_init_implode.argtypes = [P(EkoParams), P(EkoImplode), D]
+        L.eko_init_blast.argtypes = [P(EkoParams), P(EkoBlast), D]
+        L.eko_make_boundaries.argtypes = [P(EkoParams), D]
+        L.eko_compute_inv_dt.argtypes = [P(EkoParams), D]
+        L.eko_compute_inv_dt.restype = C.c_double
+        L.eko_godunov_unsplit_impl.argtypes = [P(EkoParams), D, D, D, D, D, D, C.c_double]
+        L.eko_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
+        L.eko_run.restype = C.c_int
+        L.eko_num_threads.restype = C.c_int
+        _L = L
+    return _L
+
+
+def dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def from_ek(ekp) -> EkoParams:
+    """Oracle params from the product's ek_params (same field values)."""
+    p = EkoParams()
+    for f in ("dim", "nx", "ny", "nz", "gw", "nbvar", "riemann", "mhd", "xmin", "xmax", "ymin",
+              "ymax", "zmin", "zmax", "gamma0", "cfl", "slope_type", "smallr", "smallc", "cIso"):
+        setattr(p, f, getattr(ekp, f))
+    for i in range(6):
+        p.bc[i] = ekp.bc[i]
+    lib().eko_params_init(C.byref(p))
+    return p
+
+
+def ref_shape(p: EkoParams):
+    s = (p.isize, p.jsize) + ((p.ksize,) if p.dim == 3 else ())
+    return s + (p.nbvar,)
+
+
+def implode(p: EkoParams, **kw) -> np.ndarray:
+    ip = EkoImplode(rho_out=1.0, p_out=1.0, rho_in=0.125, p_in=float(np.float32(0.14)))
+    for k, v in kw.items():
+        setattr(ip, k, v)
+    U = np.zeros(ref_shape(p))
+    lib().eko_init_implode(C.byref(p), C.byref(ip), dp(U))
+    return U
+
+
+def blast(p: EkoParams, **kw) -> np.ndarray:
+    bp = EkoBlast()
+    for k, v in kw.items():
+        setattr(bp, k, v)
+    U = np.zeros(ref_shape(p))
+    lib().eko_init_blast(C.byref(p), C.byref(bp), dp(U))
+    return U
+
+
+def run(p: EkoParams, U: np.ndarray, nStepmax: int, tEnd: float, t: float = 0.0, it: int = 0):
+    """Reference time loop; U is advanced in place. Returns (t, it, last dt)."""
+    tt, ii, dd = C.c_double(t), C.c_int(it), C.c_double()
+    rc = lib().eko_run(C.byref(p), dp(U), nStepmax, tEnd, C.byref(tt), C.byref(ii), C.byref(dd))
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return tt.value, ii.value, dd.value
+
+
+def inv_dt(p: EkoParams, U: np.ndarray) -> float:
+    return lib().eko_compute_inv_dt(C.byref(p), dp(U))
+
+
+def interior(p: EkoParams, U: np.ndarray) -> np.ndarray:
+    g = p.gw
+    if p.dim == 2:
+        return U[g:-g, g:-g, :]
+    return U[g:-g, g:-g, g:-g, :]

@@ -1,0 +1,44 @@
+"""CPU-side checks of the native library: it loads, exports every entry point declared in
+include/ek_hip.h, and rejects invalid configurations before touching a device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from euler_kokkos_amd import native
+from euler_kokkos_amd.params import make_params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:[A-Za-z_][\w\s\*]*?\s\**)(ek\w+)\s*\(", src, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = native.lib()
+    names = declared_symbols()
+    assert {"ek_create", "ek_run", "ek_step", "ek_inv_dt"} <= names
+    for n in sorted(names):
+        assert hasattr(L, n), n
+    assert set(native.EXPORTS) <= names
+
+
+@pytest.mark.parametrize("bad", [
+    dict(riemann="hll", solver="Hydro_Muscl_3D"),   # reference 3D dispatch has no HLL
+    dict(slope_type="1.5"),                          # reference leaves slopes unset
+    dict(riemann="hlld"),                            # MHD-only solver
+])
+def test_create_rejects_unsupported(bad):
+    solver = bad.pop("solver", "Hydro_Muscl_2D")
+    p = make_params(solver, 16, 16, 16 if solver.endswith("3D") else 1, **bad)
+    h = C.c_void_p()
+    rc = native.lib().ek_create(C.byref(p), C.byref(h))
+    assert rc == 1
+    assert native.lib().ek_last_error(None)
