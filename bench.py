@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Headline benchmark: the fused MUSCL-Hancock step on one MI355X.
+
+Workload (one step = one godunov_unsplit iteration of the whole grid): 3D Euler blast,
+256^3 cells on the unit cube, periodic boundaries, gamma 1.4, HLLC, MC slopes
+(slope_type 2), CFL 0.5; rho = 1, v = 0, p = 0.1 everywhere and p = 10 inside a sphere
+of radius 0.1 centred at (0.5, 0.5, 0.5). The state is resident in HBM before timing;
+the timed region is `steps` iterations of the device-resident time loop (dt computed on
+the device every step). `value` is interior cell-updates per second in FP64 (the
+bit-exact path). The same steps are then run in FP32 bulk mode and its density/energy
+L1 error against the FP64 result is reported, and the oracle (a CPU restatement of the
+reference's OpenMP step) is timed on the host cores on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps 100] [--warmup 5]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("cell-updates/s at 1 GPU, % HBM peak; batched runs/s + p50/p99 at FP32 vs FP64")
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hydro3d_step_fp64_256cube_traffic.txt")
+
+
+def device_sync() -> None:
+    """hipDeviceSynchronize (the device-wide barrier torch.cuda.synchronize() issues)."""
+    hip = C.CDLL("libamdhip64.so")
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def read_traffic():
+    """HBM bytes per step launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (see file)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def gpu_run(ekp, U0, warmup, steps):
+    from euler_kokkos_amd import native
+    ctx = native.Context(ekp)
+    try:
+        ctx.upload(U0, native.EK_LAYOUT_SOA)
+        t, it, _ = ctx.run(warmup, warmup + steps, 1e30)
+        device_sync()
+        t0 = time.perf_counter()
+        t, it, dt = ctx.run(steps, warmup + steps, 1e30, t, it)
+        device_sync()
+        wall = time.perf_counter() - t0
+        ev_ms, launches = ctx.last_run_time()
+        U = ctx.download(native.EK_LAYOUT_SOA)
+        b = ctx.budgets()
+    finally:
+        ctx.close()
+    return dict(wall=wall, ev_ms=ev_ms, launches=launches, U=U, t=t, it=it, dt=dt, budgets=b)
+
+
+def cpu_baseline(n, seconds):
+    """The oracle (CPU restatement of the reference step, OpenMP) on this host's cores."""
+    from oracle import pyoracle
+    from euler_kokkos_amd.params import make_params
+    ekp = make_params("Hydro_Muscl_3D", n, n, n, gamma0="1.4", cfl="0.5", slope_type="2",
+                      riemann="hllc", bc=(3,) * 6)
+    op = pyoracle.from_ek(ekp)
+    U = pyoracle.blast(op, radius=0.1, center_x=0.5, center_y=0.5, center_z=0.5, density_in=1.0,
+                       density_out=1.0, pressure_in=10.0, pressure_out=0.1)
+    t0 = time.perf_counter()
+    pyoracle.run(op, U, 1, 1e30)
+    one = time.perf_counter() - t0
+    steps = int(max(1, min(20, seconds // max(one, 1e-3))))
+    t0 = time.perf_counter()
+    pyoracle.run(op, U, steps, 1e30)
+    el = time.perf_counter() - t0
+    cores = pyoracle.lib().eko_num_threads()
+    cups = steps * n ** 3 / el
+    return {"value": cups, "unit": "cell-updates/s", "cores": int(cores), "kind": "port",
+            "sample": f"oracle (C restatement of the reference impl-0 OpenMP step, -O2, no FMA) "
+                      f"on the same {n}^3 blast, {steps} steps after 1 untimed step, "
+                      f"{cores} OpenMP threads",
+            "ref_style_mcell_per_s": steps * (n + 4) ** 3 / el * 1e-6}
+
+
+def copy_calibration(gib=1.0):
+    """A device-to-device copy of known size, for scaling rocprofv3 FETCH/WRITE counters."""
+    hip = C.CDLL("libamdhip64.so")
+    n = int(gib * (1 << 30))
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n}))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="grid edge (default: the 256^3 workload)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 bulk run")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build has no multi-GPU "
+              "path; see DESIGN.md for how the grid would shard.", file=sys.stderr)
+        return 2
+    if a.copy_calibration:
+        copy_calibration()
+        return 0
+
+    from euler_kokkos_amd import native, problems
+    from euler_kokkos_amd.params import make_params
+    n = a.n
+    kw = dict(gamma0="1.4", cfl="0.5", slope_type="2", riemann="hllc", bc=(3,) * 6)
+    ekp64 = make_params("Hydro_Muscl_3D", n, n, n, **kw)
+    U0 = problems.blast(ekp64, radius="0.1", density_in="1", density_out="1", pressure_in="10",
+                        pressure_out="0.1")
+    r64 = gpu_run(ekp64, U0, a.warmup, a.steps)
+    cells = n ** 3
+    value = a.steps * cells / r64["wall"]
+    avg_ms = r64["ev_ms"] / max(r64["launches"], 1)
+    alg_bytes = cells * 2 * 5 * 8
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r64["wall"] / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"3D Euler blast {n}^3 periodic, MUSCL-Hancock + HLLC, MC slopes, "
+                               f"gamma 1.4, CFL 0.5, FP64 (bit-exact path)",
+                   "grid": [n, n, n], "solver": "Hydro_Muscl_3D", "riemann": "hllc",
+                   "slope_type": 2, "cfl": 0.5, "precision": "fp64"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
+                     "kernel": "hydro3d_step<double,16,8,256,HLLC,MC>",
+                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes},
+    }
+    if not a.no_fp32:
+        ekp32 = make_params("Hydro_Muscl_3D", n, n, n, precision=native.EK_FP32, **kw)
+        r32 = gpu_run(ekp32, U0, a.warmup, a.steps)
+        avg32 = r32["ev_ms"] / max(r32["launches"], 1)
+        d64, d32 = r64["U"], r32["U"]
+        out["fp32"] = {
+            "value": a.steps * cells / r32["wall"], "ms_per_step": r32["wall"] / a.steps * 1e3,
+            "speedup_vs_fp64": r64["wall"] / r32["wall"],
+            "achieved_GBps": cells * 2 * 5 * 4 / (avg32 * 1e-3) / 1e9,
+            "l1_rho": float(np.abs(d32[0] - d64[0]).mean()),
+            "l1_E_rel": float(np.abs(d32[1] - d64[1]).mean() / np.abs(d64[1]).mean()),
+        }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, a.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -33,6 +33,9 @@ struct ek_ctx {
   long long bytes = 0;
   // captured 6-step graph (6 = lcm of the 2 state buffers and the 3 time slots)
   hipGraphExec_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr; // bracket the step launches of the last ek_run
+  double last_run_ms = 0.0;
+  int last_run_launches = 0;
   int graph_nStepmax = -1;
   double graph_tEnd = 0.0;
   void* graph_U0 = nullptr;
@@ -167,7 +170,8 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
   hipMemsetAsync(c->U[0], 0, state_bytes, c->stream);
   hipMemsetAsync(c->U[1], 0, state_bytes, c->stream);
   hipMemsetAsync(c->ts, 0, 4 * sizeof(ek::TimeSlot), c->stream);
-  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+  if ((e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+      (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
     ek_destroy(c);
     return fail(nullptr, EK_EHIP, std::string("HIP init: ") + hipGetErrorString(e));
   }
@@ -183,6 +187,10 @@ extern "C" void ek_destroy(ek_ctx* c)
     hipStreamSynchronize(c->stream);
   if (c->graph)
     hipGraphExecDestroy(c->graph);
+  if (c->ev0)
+    hipEventDestroy(c->ev0);
+  if (c->ev1)
+    hipEventDestroy(c->ev1);
   hipFree(c->U[0]);
   hipFree(c->U[1]);
   hipFree(c->ts);
@@ -391,10 +399,12 @@ extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* 
   EK_HIP(c, hipMemcpyAsync(c->ts, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   EK_HIP(c, launch_inv_dt_into(c, &c->ts[0].invdt_bits));
   const int nfull = nsteps / 6, rem = nsteps % 6;
+  if (nfull > 0 && (!c->graph || c->graph_nStepmax != nStepmax || c->graph_tEnd != tEnd ||
+                    c->graph_U0 != c->U[0]))
+    if (int r = build_graph(c, nStepmax, tEnd))
+      return r;
+  EK_HIP(c, hipEventRecord(c->ev0, c->stream));
   if (nfull > 0) {
-    if (!c->graph || c->graph_nStepmax != nStepmax || c->graph_tEnd != tEnd || c->graph_U0 != c->U[0])
-      if (int r = build_graph(c, nStepmax, tEnd))
-        return r;
     for (int g = 0; g < nfull; g++)
       EK_HIP(c, hipGraphLaunch(c->graph, c->stream));
   }
@@ -408,11 +418,16 @@ extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* 
     ctl.nStepmax = nStepmax;
     EK_HIP(c, launch_one(c, ctl, c->U[r % 2], c->U[(r + 1) % 2]));
   }
+  EK_HIP(c, hipEventRecord(c->ev1, c->stream));
   ek::TimeSlot fin;
   EK_HIP(c, hipMemcpyAsync(&fin, &c->ts[nsteps % 3], sizeof(fin), hipMemcpyDeviceToHost, c->stream));
   EK_HIP(c, hipStreamSynchronize(c->stream));
   const int ran = fin.it - *it_io;
   c->cur = ran % 2;
+  float ms = 0.f;
+  EK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_run_ms = ms;
+  c->last_run_launches = nsteps;
   *t_io = fin.t;
   *it_io = fin.it;
   if (dt_last)
@@ -455,6 +470,15 @@ extern "C" int ek_sync(ek_ctx* c)
   if (!c)
     return fail(c, EK_EINVAL, "null argument");
   EK_HIP(c, hipStreamSynchronize(c->stream));
+  return EK_OK;
+}
+
+extern "C" int ek_last_run_time(const ek_ctx* c, double* ms, int* launches)
+{
+  if (!c || !ms || !launches)
+    return EK_EINVAL;
+  *ms = c->last_run_ms;
+  *launches = c->last_run_launches;
   return EK_OK;
 }
 

@@ -375,268 +375,6 @@ __global__ __launch_bounds__(NT) void hydro2d_step(const T* __restrict__ Uin, T*
 }
 
 // ------------------------------------------------------------------------------------
-// 3D: one block per TX x TY column tile and z-chunk of CZ planes, marching along z.
-// A 3-plane ring of primitives (2-cell xy halo) lives in LDS; the z-face states and the
-// partially updated cell of the previous plane stay in the owning thread's registers.
-// ------------------------------------------------------------------------------------
-template <class T, int TX, int TY, int NT, int RS, int ST>
-__global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T* __restrict__ Uout,
-                                                   Geom<T> geo, Gas<T> gas, StepCtl ctl, int CZ)
-{
-  static_assert(NT >= TX * TY, "one owner thread per tile column");
-  constexpr int NV = 5;
-  constexpr int QX = TX + 4, QY = TY + 4, QN = QX * QY;
-  __shared__ T sQ[3][NV][QN];
-  __shared__ T sX[2][NV][TY][TX + 1];
-  __shared__ T sY[2][NV][TY + 1][TX];
-  __shared__ double sred[NT / 64];
-
-  const int tid = threadIdx.x;
-  const bool first_block = blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
-  double dt;
-  TimeSlot cur;
-  if (!step_begin(ctl, first_block, dt, cur))
-    return;
-  const T dtdx[3] = { T(dt / geo.dx), T(dt / geo.dy), T(dt / geo.dz) };
-  const int nx = geo.nx, ny = geo.ny, nz = geo.nz;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
-  const int k0 = blockIdx.z * CZ;
-  const int k1 = min(k0 + CZ, nz);
-
-  auto load_plane = [&](int kp) {
-    const int s = ((kp % 3) + 3) % 3;
-    bool nz_;
-    const int sk = bc_map(kp, nz, geo.bc[4], geo.bc[5], nz_);
-    for (int w = tid; w < QN; w += NT) {
-      const int qx = w % QX, qy = w / QX;
-      bool nx_, ny_;
-      const int si = bc_map(i0 - 2 + qx, nx, geo.bc[0], geo.bc[1], nx_);
-      const int sj = bc_map(j0 - 2 + qy, ny, geo.bc[2], geo.bc[3], ny_);
-      const long long o = (long long)sk * geo.plane + (long long)sj * nx + si;
-      T u[NV];
-#pragma unroll
-      for (int v = 0; v < NV; v++)
-        u[v] = Uin[v * geo.vstride + o];
-      if (nx_)
-        u[IU] = u[IU] * T(-1);
-      if (ny_)
-        u[IV] = u[IV] * T(-1);
-      if (nz_)
-        u[IW] = u[IW] * T(-1);
-      T q[NV];
-      prims<T, 3>(gas, u, q);
-#pragma unroll
-      for (int v = 0; v < NV; v++)
-        sQ[s][v][w] = q[v];
-    }
-  };
-
-  // slopes of the cell at Q-grid offset c in plane kp (ring slots of kp-1, kp, kp+1)
-  auto slopes_at = [&](int kp, int c, T* q, T (*dq)[5]) {
-    const int sm = ((kp - 1) % 3 + 3) % 3, s0 = ((kp % 3) + 3) % 3, sp = ((kp + 1) % 3 + 3) % 3;
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      q[v] = sQ[s0][v][c];
-      if constexpr (ST == 0) {
-        dq[0][v] = T(0);
-        dq[1][v] = T(0);
-        dq[2][v] = T(0);
-      } else {
-        dq[0][v] = slope1<T, ST>(q[v], sQ[s0][v][c + 1], sQ[s0][v][c - 1]);
-        dq[1][v] = slope1<T, ST>(q[v], sQ[s0][v][c + QX], sQ[s0][v][c - QX]);
-        dq[2][v] = slope1<T, ST>(q[v], sQ[sp][v][c], sQ[sm][v][c]);
-      }
-    }
-  };
-
-  const bool owner = tid < TX * TY;
-  const int ox = tid % TX, oy = tid / TX;
-  const int ogi = i0 + ox, ogj = j0 + oy;
-  const bool ovalid = owner && ogi < nx && ogj < ny;
-  const int oc = (oy + 2) * QX + (ox + 2);
-  T qzm[NV];   // max-z face state of the owner's cell in the previous plane
-  T acc[NV];   // owner's cell of the previous plane, updated up to its min-z face
-  double local = 0.0;
-
-  auto zflux = [&](const T* qzp, T* fz) {
-    T ql[NV], qr[NV], f[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      ql[v] = qzm[v];
-      qr[v] = qzp[v];
-    }
-    T t;
-    t = ql[IU]; ql[IU] = ql[IW]; ql[IW] = t;
-    t = qr[IU]; qr[IU] = qr[IW]; qr[IW] = t;
-    riemann<T, NV, RS>(gas, ql, qr, f);
-    fz[ID] = f[ID] * dtdx[2];
-    fz[IP] = f[IP] * dtdx[2];
-    fz[IU] = f[IW] * dtdx[2];
-    fz[IV] = f[IV] * dtdx[2];
-    fz[IW] = f[IU] * dtdx[2];
-  };
-
-  auto finish_prev = [&](int kprev, const T* fz) {
-    // last term of UpdateFunctor3D for plane kprev: -Fz(k+1); then store and reduce CFL
-    T u[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++) {
-      acc[v] -= fz[v];
-      u[v] = acc[v];
-    }
-    if (ovalid) {
-      const long long o = (long long)kprev * geo.plane + (long long)ogj * nx + ogi;
-#pragma unroll
-      for (int v = 0; v < NV; v++)
-        Uout[v * geo.vstride + o] = u[v];
-      const double s = (double)cell_inv_dt<T, 3>(gas, u, geo.tdx, geo.tdy, geo.tdz, geo.idx,
-                                                 geo.idy, geo.idz, geo.pow2 != 0);
-      local = (s > local || !(s == s)) ? s : local;
-    }
-  };
-
-  load_plane(k0 - 2);
-  load_plane(k0 - 1);
-  constexpr int PT = TX * TY, PH = 2 * TY + 2 * TX;
-  constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1);
-
-  for (int p = k0 - 1; p <= k1; p++) {
-    load_plane(p + 1);
-    __syncthreads();
-    if (p == k0 - 1) {
-      if (owner) {
-        T q[NV], dq[3][5], fm[3][5], fp[3][5];
-        slopes_at(p, oc, q, dq);
-        trace_cell<T, 3>(gas, q, dq, dtdx, 0x20, fm, fp);
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-          qzm[v] = fm[2][v];
-      }
-      __syncthreads();
-      continue;
-    }
-    if (p == k1) {
-      if (owner) {
-        T q[NV], dq[3][5], fm[3][5], fp[3][5], fz[NV];
-        slopes_at(p, oc, q, dq);
-        trace_cell<T, 3>(gas, q, dq, dtdx, 0x10, fm, fp);
-        zflux(fp[2], fz);
-        finish_prev(p - 1, fz);
-      }
-      break;
-    }
-    // plane p: slopes + states for the tile and its halo cross
-    T fzp[NV];
-    for (int w = tid; w < PT + PH; w += NT) {
-      int cx, cy;
-      unsigned want;
-      if (w < PT) {
-        cx = w % TX;
-        cy = w / TX;
-        want = 0x3F;
-      } else {
-        const int h = w - PT;
-        if (h < TY) {
-          cx = -1; cy = h; want = 0x2;
-        } else if (h < 2 * TY) {
-          cx = TX; cy = h - TY; want = 0x1;
-        } else if (h < 2 * TY + TX) {
-          cx = h - 2 * TY; cy = -1; want = 0x8;
-        } else {
-          cx = h - 2 * TY - TX; cy = TY; want = 0x4;
-        }
-      }
-      const int c = (cy + 2) * QX + (cx + 2);
-      T q[NV], dq[3][5], fm[3][5], fp[3][5];
-      slopes_at(p, c, q, dq);
-      trace_cell<T, 3>(gas, q, dq, dtdx, want, fm, fp);
-      if (want & 0x2) {
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-          sX[0][v][cy][cx + 1] = fm[0][v];
-      }
-      if (want & 0x1) {
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-          sX[1][v][cy][cx] = fp[0][v];
-      }
-      if (want & 0x8) {
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-          sY[0][v][cy + 1][cx] = fm[1][v];
-      }
-      if (want & 0x4) {
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-          sY[1][v][cy][cx] = fp[1][v];
-      }
-      if (w < PT) {
-        // w == tid here: the owner solves the z face below its cell and carries its max-z state
-        zflux(fp[2], fzp);
-        if (p > k0)
-          finish_prev(p - 1, fzp);
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-          qzm[v] = fm[2][v];
-      }
-    }
-    __syncthreads();
-    for (int w = tid; w < NFX + NFY; w += NT) {
-      T ql[NV], qr[NV], f[NV];
-      if (w < NFX) {
-        const int fx = w % (TX + 1), fy = w / (TX + 1);
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          ql[v] = sX[0][v][fy][fx];
-          qr[v] = sX[1][v][fy][fx];
-        }
-        riemann<T, NV, RS>(gas, ql, qr, f);
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-          sX[0][v][fy][fx] = f[v] * dtdx[0];
-      } else {
-        const int h = w - NFX;
-        const int fx = h % TX, fy = h / TX;
-#pragma unroll
-        for (int v = 0; v < NV; v++) {
-          ql[v] = sY[0][v][fy][fx];
-          qr[v] = sY[1][v][fy][fx];
-        }
-        T t;
-        t = ql[IU]; ql[IU] = ql[IV]; ql[IV] = t;
-        t = qr[IU]; qr[IU] = qr[IV]; qr[IV] = t;
-        riemann<T, NV, RS>(gas, ql, qr, f);
-        sY[0][ID][fy][fx] = f[ID] * dtdx[1];
-        sY[0][IP][fy][fx] = f[IP] * dtdx[1];
-        sY[0][IU][fy][fx] = f[IV] * dtdx[1];
-        sY[0][IV][fy][fx] = f[IU] * dtdx[1];
-        sY[0][IW][fy][fx] = f[IW] * dtdx[1];
-      }
-    }
-    __syncthreads();
-    if (owner) {
-      // UpdateFunctor3D order: +Fx(i) -Fx(i+1) +Fy(j) -Fy(j+1) +Fz(k) (-Fz(k+1) next plane)
-      const long long o = (long long)p * geo.plane + (long long)(ogj < ny ? ogj : 0) * nx +
-                          (ogi < nx ? ogi : 0);
-#pragma unroll
-      for (int v = 0; v < NV; v++) {
-        T a = ovalid ? Uin[v * geo.vstride + o] : T(0);
-        a += sX[0][v][oy][ox];
-        a -= sX[0][v][oy][ox + 1];
-        a += sY[0][v][oy][ox];
-        a -= sY[0][v][oy + 1][ox];
-        a += fzp[v];
-        acc[v] = a;
-      }
-    }
-    __syncthreads();
-  }
-  block_max_to<NT>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
-  if (first_block && tid == 0)
-    step_end_bookkeeping(ctl, cur, dt);
-}
-
-// ------------------------------------------------------------------------------------
 // CFL reduction over the current state (ComputeDtFunctor2D/3D), into slot `slot`.
 // ------------------------------------------------------------------------------------
 template <class T, int DIM, int NT>

@@ -1,0 +1,388 @@
+// ek_hydro3d.h -- fused 3D hydro MUSCL-Hancock step for gfx950, marching along z.
+//
+// One block owns a TX x TY column tile and a chunk of z planes. Every thread owns one
+// position of the tile's (TX+4) x (TY+4) primitive grid for the whole march and keeps
+// that column's primitives at planes p-1, p, p+1 in registers, so the z slope never
+// touches LDS; only plane p (for the x/y slopes) and the face states live in LDS
+// (about 42 KB for 16x8 in FP64, three blocks per CU).
+//
+// Per plane p:  A  load U(p+1), primitives into registers (all positions)
+//               B  slopes + Hancock trace at the tile and its 1-wide halo cross;
+//                  face states to LDS (x, y, and the tile's z faces)
+//               W  primitives of plane p+1 into the LDS plane
+//               C  Riemann problems: (TX+1)TY x-faces, TX(TY+1) y-faces, TXTY z-faces,
+//                  spread over the waves so that each wave's total work is even
+//               D  update of plane p (UpdateFunctor3D order) and the -Fz(p) term plus the
+//                  CFL contribution of plane p-1
+// The arithmetic per cell, face and update is exactly the reference's
+// (src/muscl/HydroRunFunctors3D.h:271-820); only where and how often it is evaluated
+// changes: slopes once per cell (the reference recomputes them for 4 cells), each face
+// solved once.
+#pragma once
+
+#include "ek_hydro.h"
+
+namespace ek {
+
+template <int TX, int TY, int NT>
+struct Col3 {
+  static constexpr int QX = TX + 4, QY = TY + 4, QN = QX * QY;
+  static constexpr int PT = TX * TY;               // tile cells (all six face states)
+  static constexpr int PH = 2 * TX + 2 * TY;       // halo cross (one face state each)
+  static constexpr int PO = 2 * TX + 2 * TY + 4;   // primitives only (slope stencils)
+  static constexpr int NPOS = PT + PH + PO;
+  static constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NFZ = PT;
+  static constexpr int NF = NFX + NFY + NFZ;
+  static constexpr int NW = NT / 64;
+  static_assert(NT % 64 == 0, "whole waves");
+  static_assert(NPOS <= NT, "one primitive column per thread");
+};
+
+// Static split of the face list over the waves: each 64-face round goes to the wave whose
+// estimated work (slopes/trace of its lanes, update, rounds already given) is smallest.
+template <int TX, int TY, int NT>
+struct FacePlan {
+  int start[NT / 64];
+  int count[NT / 64];
+};
+
+template <int TX, int TY, int NT>
+constexpr FacePlan<TX, TY, NT> make_face_plan()
+{
+  using C = Col3<TX, TY, NT>;
+  FacePlan<TX, TY, NT> fp{};
+  int load[C::NW] = {};
+  int rounds[C::NW] = {};
+  for (int w = 0; w < C::NW; w++) {
+    const int lo = 64 * w, hi = 64 * w + 64;
+    const bool tile = lo < C::PT;
+    const bool cross = hi > C::PT && lo < C::PT + C::PH;
+    load[w] = (tile ? 34 + 15 : (cross ? 29 : 0));
+  }
+  const int R = (C::NF + 63) / 64;
+  for (int r = 0; r < R; r++) {
+    int best = 0;
+    for (int w = 1; w < C::NW; w++)
+      if (load[w] < load[best])
+        best = w;
+    load[best] += 20;
+    rounds[best] += 1;
+  }
+  // contiguous ranges, lightest-loaded waves first so the partial round lands last
+  int order[C::NW] = {};
+  for (int w = 0; w < C::NW; w++)
+    order[w] = C::NW - 1 - w;
+  int s = 0;
+  for (int i = 0; i < C::NW; i++) {
+    const int w = order[i];
+    fp.start[w] = s;
+    int c = rounds[w] * 64;
+    if (s + c > C::NF)
+      c = C::NF - s;
+    if (c < 0)
+      c = 0;
+    fp.count[w] = c;
+    s += c;
+  }
+  return fp;
+}
+
+// Q-grid coordinates of position t and the face states it must produce
+// (bit 2d: min face along d, bit 2d+1: max face along d).
+template <int TX, int TY, int NT>
+__device__ __forceinline__ bool col_pos(int t, int& qx, int& qy, unsigned& want)
+{
+  using C = Col3<TX, TY, NT>;
+  if (t < C::PT) {
+    qx = t % TX + 2;
+    qy = t / TX + 2;
+    want = 0x3F;
+    return true;
+  }
+  if (t < C::PT + C::PH) {
+    const int h = t - C::PT;
+    if (h < TY) {
+      qx = 1; qy = h + 2; want = 0x2;
+    } else if (h < 2 * TY) {
+      qx = TX + 2; qy = h - TY + 2; want = 0x1;
+    } else if (h < 2 * TY + TX) {
+      qx = h - 2 * TY + 2; qy = 1; want = 0x8;
+    } else {
+      qx = h - 2 * TY - TX + 2; qy = TY + 2; want = 0x4;
+    }
+    return true;
+  }
+  if (t < C::NPOS) {
+    const int o = t - C::PT - C::PH;
+    want = 0;
+    if (o < TY) {
+      qx = 0; qy = o + 2;
+    } else if (o < 2 * TY) {
+      qx = TX + 3; qy = o - TY + 2;
+    } else if (o < 2 * TY + TX) {
+      qx = o - 2 * TY + 2; qy = 0;
+    } else if (o < 2 * TY + 2 * TX) {
+      qx = o - 2 * TY - TX + 2; qy = TY + 3;
+    } else {
+      const int c = o - 2 * TY - 2 * TX;
+      qx = (c & 1) ? TX + 2 : 1;
+      qy = (c & 2) ? TY + 2 : 1;
+    }
+    return true;
+  }
+  qx = qy = 0;
+  want = 0;
+  return false;
+}
+
+// Component permutations of the y and z solves: IU <-> IV, IU <-> IW (self-inverse).
+__device__ __forceinline__ constexpr int rot_y(int v)
+{
+  return v == IU ? IV : (v == IV ? IU : v);
+}
+__device__ __forceinline__ constexpr int rot_z(int v)
+{
+  return v == IU ? IW : (v == IW ? IU : v);
+}
+
+template <class T, int TX, int TY, int NT, int RS, int ST>
+__global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T* __restrict__ Uout,
+                                                   Geom<T> geo, Gas<T> gas, StepCtl ctl, int CZ,
+                                                   int ntx, int nty, int nchunk)
+{
+  using C = Col3<TX, TY, NT>;
+  constexpr int NV = 5;
+  constexpr int QX = C::QX, QN = C::QN, PT = C::PT;
+  constexpr int NFX = C::NFX, NFY = C::NFY, NF = C::NF;
+  constexpr FacePlan<TX, TY, NT> plan = make_face_plan<TX, TY, NT>();
+  __shared__ T sQ[NV][QN];
+  // Face states, rotated so that IU holds the normal velocity (the swap the reference does
+  // before each y/z solve, HydroRunFunctors3D.h:597,692): [0] left state, [1] right state.
+  // Faces: x (fy*(TX+1)+fx), then y (NFX + fy*TX+fx), then z (NFX+NFY + tile cell).
+  // After the solve, [0][.][f] holds the face flux times dt/dx, still rotated.
+  __shared__ T sF[2][NV][NF];
+  __shared__ double sred[NT / 64];
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  // XCD-aware tile order: blocks are dispatched round-robin over the 8 XCDs, so give each
+  // XCD a contiguous range of tiles (neighbours share halo lines in that XCD's L2).
+  const int nblk = ntx * nty * nchunk;
+  const int b = blockIdx.x;
+  const int per = nblk / 8;
+  const int lb = (per * 8 == nblk) ? (b % 8) * per + b / 8 : b;
+  const int bx = lb % ntx, by = (lb / ntx) % nty, bz = lb / (ntx * nty);
+  const bool first_block = b == 0;
+
+  double dt;
+  TimeSlot cur;
+  if (!step_begin(ctl, first_block, dt, cur))
+    return;
+  const T dtdx[3] = { T(dt / geo.dx), T(dt / geo.dy), T(dt / geo.dz) };
+  const int nx = geo.nx, ny = geo.ny, nz = geo.nz;
+  const int i0 = bx * TX, j0 = by * TY;
+  const int k0 = bz * CZ;
+  const int k1 = min(k0 + CZ, nz);
+
+  int qx, qy;
+  unsigned want;
+  const bool haspos = col_pos<TX, TY, NT>(tid, qx, qy, want);
+  const int c = qy * QX + qx;
+  bool negx, negy;
+  const int si = bc_map(i0 - 2 + qx, nx, geo.bc[0], geo.bc[1], negx);
+  const int sj = bc_map(j0 - 2 + qy, ny, geo.bc[2], geo.bc[3], negy);
+  const long long col = (long long)sj * nx + si;
+  const bool owner = tid < PT;
+  const int ox = tid % TX, oy = tid / TX;
+  const bool ovalid = owner && (i0 + ox) < nx && (j0 + oy) < ny;
+  const long long ocol = (long long)(j0 + oy) * nx + (i0 + ox);
+
+  T qp[NV], qc[NV], qn[NV];  // this column's primitives at planes p-1, p, p+1
+  T qzm[NV];                 // owner: max-z face state of its cell in plane p-1
+  T acc[NV];                 // owner: its cell in plane p-1, updated up to +Fz(p-1)
+  double local = 0.0;
+
+  // raw conservative state of this column at plane kp, ghost planes through the boundary map
+  auto fetch = [&](int kp, T* u) {
+    bool negz;
+    const int sk = bc_map(kp, nz, geo.bc[4], geo.bc[5], negz);
+    const long long o = (long long)sk * geo.plane + col;
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      u[v] = Uin[v * geo.vstride + o];
+    if (negx)
+      u[IU] = u[IU] * T(-1);
+    if (negy)
+      u[IV] = u[IV] * T(-1);
+    if (negz)
+      u[IW] = u[IW] * T(-1);
+  };
+
+  // software pipeline: upre holds U(p+1) when iteration p starts; the loads for p+2 are
+  // issued right after it is consumed and complete behind phases B-D
+  T upre[NV], ucur[NV];
+  if (haspos) {
+    fetch(k0 - 2, upre);
+    prims<T, 3>(gas, upre, qp);
+    fetch(k0 - 1, upre);
+    prims<T, 3>(gas, upre, qc);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      sQ[v][c] = qc[v];
+    fetch(k0, upre);
+  }
+  __syncthreads();
+
+  for (int p = k0 - 1; p <= k1; p++) {
+    const bool warm = p == k0 - 1, last = p == k1;
+    // A: primitives of plane p+1 for this column; prefetch p+2; the owner's U(p) for D
+    if (haspos) {
+      prims<T, 3>(gas, upre, qn);
+      if (p + 2 <= k1 + 1)
+        fetch(p + 2, upre);
+    }
+    if (ovalid && !warm && !last) {
+      const long long o = (long long)p * geo.plane + ocol;
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        ucur[v] = Uin[v * geo.vstride + o];
+    }
+    // B: slopes and face states of plane p
+    if (want) {
+      unsigned w = want;
+      if (warm || last)
+        w = owner ? (warm ? 0x20u : 0x10u) : 0u;
+      if (w) {
+        T dq[3][5], fm[3][5], fp[3][5];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          if constexpr (ST == 0) {
+            dq[0][v] = T(0);
+            dq[1][v] = T(0);
+            dq[2][v] = T(0);
+          } else {
+            dq[2][v] = slope1<T, ST>(qc[v], qn[v], qp[v]);
+            dq[0][v] = slope1<T, ST>(qc[v], sQ[v][c + 1], sQ[v][c - 1]);
+            dq[1][v] = slope1<T, ST>(qc[v], sQ[v][c + QX], sQ[v][c - QX]);
+          }
+        }
+        trace_cell<T, 3>(gas, qc, dq, dtdx, w, fm, fp);
+        const int cx = qx - 2, cy = qy - 2;
+        // rotated stores: component IU <- normal velocity of the face direction
+        if (w & 0x2) {
+          const int f = cy * (TX + 1) + cx + 1;
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            sF[0][v][f] = fm[0][v];
+        }
+        if (w & 0x1) {
+          const int f = cy * (TX + 1) + cx;
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            sF[1][v][f] = fp[0][v];
+        }
+        if (w & 0x8) {
+          const int f = NFX + (cy + 1) * TX + cx;
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            sF[0][v][f] = fm[1][rot_y(v)];
+        }
+        if (w & 0x4) {
+          const int f = NFX + cy * TX + cx;
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            sF[1][v][f] = fp[1][rot_y(v)];
+        }
+        if (w & 0x10) {
+          const int f = NFX + NFY + tid;
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            sF[0][v][f] = qzm[rot_z(v)];
+            sF[1][v][f] = fp[2][rot_z(v)];
+          }
+        }
+        if (w & 0x20) {
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            qzm[v] = fm[2][v];
+        }
+      }
+    }
+    __syncthreads();
+    // W: plane p+1 becomes the LDS plane (every read of plane p is done)
+    if (haspos && !last) {
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        sQ[v][c] = qn[v];
+    }
+    // C: Riemann problems of plane p's faces
+    if (!warm) {
+      int fb, fe;
+      if (last) {
+        fb = NFX + NFY + tid;
+        fe = NF;
+      } else {
+        fb = plan.start[wave] + lane;
+        fe = plan.start[wave] + plan.count[wave];
+      }
+      const int stride = last ? NT : 64;
+      for (int f = fb; f < fe; f += stride) {
+        T ql[NV], qr[NV], fl[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          ql[v] = sF[0][v][f];
+          qr[v] = sF[1][v][f];
+        }
+        riemann<T, NV, RS>(gas, ql, qr, fl);
+        const T dtd = f < NFX ? dtdx[0] : (f < NFX + NFY ? dtdx[1] : dtdx[2]);
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+          sF[0][v][f] = fl[v] * dtd;
+      }
+    }
+    __syncthreads();
+    // D: finish plane p-1 (-Fz(p)), accumulate plane p (+Fx(i) -Fx(i+1) +Fy(j) -Fy(j+1) +Fz(p))
+    if (owner && !warm) {
+      if (p > k0) {
+        T u[NV];
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          acc[v] -= sF[0][rot_z(v)][NFX + NFY + tid];
+          u[v] = acc[v];
+        }
+        if (ovalid) {
+          const long long o = (long long)(p - 1) * geo.plane + ocol;
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            Uout[v * geo.vstride + o] = u[v];
+          const double s = (double)cell_inv_dt<T, 3>(gas, u, geo.tdx, geo.tdy, geo.tdz, geo.idx,
+                                                     geo.idy, geo.idz, geo.pow2 != 0);
+          local = (s > local || !(s == s)) ? s : local;
+        }
+      }
+      if (!last) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+          T a = ovalid ? ucur[v] : T(0);
+          a += sF[0][v][oy * (TX + 1) + ox];
+          a -= sF[0][v][oy * (TX + 1) + ox + 1];
+          a += sF[0][rot_y(v)][NFX + oy * TX + ox];
+          a -= sF[0][rot_y(v)][NFX + (oy + 1) * TX + ox];
+          a += sF[0][rot_z(v)][NFX + NFY + tid];
+          acc[v] = a;
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      qp[v] = qc[v];
+      qc[v] = qn[v];
+    }
+  }
+  block_max_to<NT>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
+  if (first_block && tid == 0)
+    step_end_bookkeeping(ctl, cur, dt);
+}
+
+} // namespace ek

@@ -4,18 +4,19 @@
 #pragma once
 
 #include "ek_hydro.h"
+#include "ek_hydro3d.h"
 
 namespace ek {
 
 // Tile shapes. 2D: 16x16 cells per block (256 threads). 3D: 16x8 column tiles marching
-// over z-chunks (128 threads, one owner thread per column).
+// over z-chunks (256 threads: one per primitive column of the tile and its 2-cell halo).
 constexpr int T2X = 16, T2Y = 16, N2T = 256;
-constexpr int T3X = 16, T3Y = 8, N3T = 128;
+constexpr int T3X = 16, T3Y = 8, N3T = 256;
 
 template <class T>
 using Step2Fn = void (*)(const T*, T*, Geom<T>, Gas<T>, StepCtl);
 template <class T>
-using Step3Fn = void (*)(const T*, T*, Geom<T>, Gas<T>, StepCtl, int);
+using Step3Fn = void (*)(const T*, T*, Geom<T>, Gas<T>, StepCtl, int, int, int, int);
 
 template <class T, int RS>
 inline Step2Fn<T> pick2(int st)
@@ -57,8 +58,10 @@ inline hipError_t launch_step(int dim, int rs, int st, const T* Uin, T* Uout, co
       case RS_LLF: f = pick3<T, RS_LLF>(st); break;
       default: f = pick3<T, RS_HLLC>(st); break;
     }
-    dim3 grid((geo.nx + T3X - 1) / T3X, (geo.ny + T3Y - 1) / T3Y, (geo.nz + cz - 1) / cz);
-    hipLaunchKernelGGL(f, grid, dim3(N3T), 0, s, Uin, Uout, geo, gas, ctl, cz);
+    const int ntx = (geo.nx + T3X - 1) / T3X, nty = (geo.ny + T3Y - 1) / T3Y;
+    const int nch = (geo.nz + cz - 1) / cz;
+    hipLaunchKernelGGL(f, dim3(ntx * nty * nch), dim3(N3T), 0, s, Uin, Uout, geo, gas, ctl, cz, ntx,
+                       nty, nch);
   }
   return hipGetLastError();
 }

@@ -1,0 +1,124 @@
+// Problems.cpp -- see Problems.h.
+#include "Problems.h"
+
+namespace euler_kokkos {
+
+namespace {
+struct Shape {
+  int nx, ny, nz, nv;
+  long long cells;
+  long long at(int v, int i, int j, int k) const { return ((long long)v * nz + k) * ny * (long long)nx + (long long)j * nx + i; }
+};
+Shape shape_of(const HydroParams& p)
+{
+  Shape s;
+  s.nx = p.nx;
+  s.ny = p.ny;
+  s.nz = p.dimType == THREE_D ? p.nz : 1;
+  s.nv = p.nbvar;
+  s.cells = (long long)s.nx * s.ny * s.nz;
+  return s;
+}
+} // namespace
+
+void init_implode(const HydroParams& p, const ConfigMap& c, std::vector<double>& U)
+{
+  const double rho_out = c.getFloat("implode", "density_outer", 1.0);
+  const double p_out = c.getFloat("implode", "pressure_outer", 1.0);
+  const double u_out = c.getFloat("implode", "vx_outer", 0.0);
+  const double v_out = c.getFloat("implode", "vy_outer", 0.0);
+  const double w_out = c.getFloat("implode", "vz_outer", 0.0);
+  const double rho_in = c.getFloat("implode", "density_inner", 0.125);
+  const double p_in = c.getFloat("implode", "pressure_inner", 0.14);
+  const double u_in = c.getFloat("implode", "vx_inner", 0.0);
+  const double v_in = c.getFloat("implode", "vy_inner", 0.0);
+  const double w_in = c.getFloat("implode", "vz_inner", 0.0);
+  const int shape = (int)c.getInteger("implode", "shape_region", 0);
+  const Shape s = shape_of(p);
+  U.assign((size_t)(s.cells * s.nv), 0.0);
+  const int gw = p.ghostWidth;
+  const double gamma0 = p.settings.gamma0;
+  for (int k = 0; k < s.nz; k++)
+    for (int j = 0; j < s.ny; j++)
+      for (int i = 0; i < s.nx; i++) {
+        // x = xmin + dx/2 + (i_padded - ghostWidth)*dx with i_padded = i + gw
+        const double x = p.xmin + p.dx / 2 + ((i + gw) - gw) * p.dx;
+        const double y = p.ymin + p.dy / 2 + ((j + gw) - gw) * p.dy;
+        bool out;
+        if (p.dimType == TWO_D) {
+          if (shape == 1)
+            out = x + y * y > 0.5 && x + y * y < 1.5;
+          else
+            out = x + y > (p.xmin + p.xmax) / 2. + p.ymin;
+          U[s.at(ID, i, j, k)] = out ? rho_out : rho_in;
+          U[s.at(IP, i, j, k)] = out ? p_out / (gamma0 - 1.0) + 0.5 * rho_out * (u_out * u_out + v_out * v_out)
+                                     : p_in / (gamma0 - 1.0) + 0.5 * rho_in * (u_in * u_in + v_in * v_in);
+          U[s.at(IU, i, j, k)] = out ? u_out : u_in;
+          U[s.at(IV, i, j, k)] = out ? v_out : v_in;
+        } else {
+          const double z = p.zmin + p.dz / 2 + ((k + gw) - gw) * p.dz;
+          if (shape == 1)
+            out = x + y + z > 0.5 && x + y + z < 2.5;
+          else
+            out = x + y + z > (p.xmin + p.xmax) / 2. + p.ymin + p.zmin;
+          U[s.at(ID, i, j, k)] = out ? rho_out : rho_in;
+          U[s.at(IP, i, j, k)] =
+            out ? p_out / (gamma0 - 1.0) + 0.5 * rho_out * (u_out * u_out + v_out * v_out + w_out * w_out)
+                : p_in / (gamma0 - 1.0) + 0.5 * rho_in * (u_in * u_in + v_in * v_in + w_in * w_in);
+          // the reference stores the velocity in the momentum slots (harmless at rest)
+          U[s.at(IU, i, j, k)] = out ? u_out : u_in;
+          U[s.at(IV, i, j, k)] = out ? v_out : v_in;
+          U[s.at(IW, i, j, k)] = out ? w_out : w_in;
+        }
+      }
+}
+
+void init_blast(const HydroParams& p, const ConfigMap& c, std::vector<double>& U)
+{
+  const double xmin = c.getFloat("mesh", "xmin", 0.0), ymin = c.getFloat("mesh", "ymin", 0.0);
+  const double zmin = c.getFloat("mesh", "zmin", 0.0), xmax = c.getFloat("mesh", "xmax", 1.0);
+  const double ymax = c.getFloat("mesh", "ymax", 1.0), zmax = c.getFloat("mesh", "zmax", 1.0);
+  const double radius = c.getFloat("blast", "radius", (xmin + xmax) / 2.0 / 10);
+  const double cx = c.getFloat("blast", "center_x", (xmin + xmax) / 2);
+  const double cy = c.getFloat("blast", "center_y", (ymin + ymax) / 2);
+  const double cz = c.getFloat("blast", "center_z", (zmin + zmax) / 2);
+  const double din = c.getFloat("blast", "density_in", 1.0);
+  const double dout = c.getFloat("blast", "density_out", 1.2);
+  const double pin = c.getFloat("blast", "pressure_in", 10.0);
+  const double pout = c.getFloat("blast", "pressure_out", 0.1);
+  const Shape s = shape_of(p);
+  U.assign((size_t)(s.cells * s.nv), 0.0);
+  const int gw = p.ghostWidth;
+  const double gamma0 = p.settings.gamma0;
+  const double radius2 = radius * radius;
+  for (int k = 0; k < s.nz; k++)
+    for (int j = 0; j < s.ny; j++)
+      for (int i = 0; i < s.nx; i++) {
+        const double x = p.xmin + p.dx / 2 + ((i + gw) - gw) * p.dx;
+        const double y = p.ymin + p.dy / 2 + ((j + gw) - gw) * p.dy;
+        double d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy);
+        if (p.dimType == THREE_D) {
+          const double z = p.zmin + p.dz / 2 + ((k + gw) - gw) * p.dz;
+          d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+        }
+        const bool in = d2 < radius2;
+        U[s.at(ID, i, j, k)] = in ? din : dout;
+        U[s.at(IP, i, j, k)] = in ? pin / (gamma0 - 1.0) : pout / (gamma0 - 1.0);
+      }
+}
+
+bool init_problem(const std::string& problem, const HydroParams& params, const ConfigMap& configMap,
+                  std::vector<double>& U)
+{
+  if (problem == "implode") {
+    init_implode(params, configMap, U);
+    return true;
+  }
+  if (problem == "blast") {
+    init_blast(params, configMap, U);
+    return true;
+  }
+  return false;
+}
+
+} // namespace euler_kokkos

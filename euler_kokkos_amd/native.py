@@ -20,7 +20,7 @@ RIEMANN = {"approx": 0, "llf": 1, "hll": 2, "hllc": 3, "hlld": 4}
 
 # Exported symbols declared by include/ek_hip.h (checked by the CPU test-suite).
 EXPORTS = ("ek_create", "ek_destroy", "ek_last_error", "ek_upload", "ek_download", "ek_inv_dt",
-           "ek_step", "ek_run", "ek_budgets", "ek_sync", "ek_device_bytes")
+           "ek_step", "ek_run", "ek_budgets", "ek_sync", "ek_device_bytes", "ek_last_run_time")
 
 
 class EkParams(C.Structure):
@@ -65,6 +65,8 @@ def lib() -> C.CDLL:
                          P(C.c_double)]
     L.ek_budgets.argtypes = [C.c_void_p, P(C.c_double)]
     L.ek_sync.argtypes = [C.c_void_p]
+    L.ek_last_run_time.argtypes = [C.c_void_p, P(C.c_double), P(C.c_int)]
+    L.ek_last_run_time.restype = C.c_int
     L.ek_device_bytes.argtypes = [C.c_void_p]
     L.ek_device_bytes.restype = C.c_longlong
     for n in ("ek_upload", "ek_download", "ek_inv_dt", "ek_step", "ek_run", "ek_budgets", "ek_sync"):
@@ -158,6 +160,12 @@ class Context:
 
     def sync(self) -> None:
         self._check(self._L.ek_sync(self.h), "ek_sync")
+
+    def last_run_time(self):
+        """(ms, launches) of the step launches of the last run(), from HIP events."""
+        ms, n = C.c_double(), C.c_int()
+        self._check(self._L.ek_last_run_time(self.h, C.byref(ms), C.byref(n)), "ek_last_run_time")
+        return ms.value, n.value
 
     def device_bytes(self) -> int:
         return int(self._L.ek_device_bytes(self.h))

@@ -100,6 +100,12 @@ int ek_budgets(ek_ctx* ctx, double* sums);
 /* Blocks until all work queued on the context's stream has finished. */
 int ek_sync(ek_ctx* ctx);
 
+/* Device time of the step launches of the last ek_run, from HIP events recorded on the
+ * context's stream around them (in ms), and the number of launches it covered.
+ * (No reference counterpart: the reference times whole phases on the host,
+ * src/shared/SolverBase.h:35-45.) */
+int ek_last_run_time(const ek_ctx* ctx, double* ms, int* launches);
+
 /* Bytes of device memory held by the context. */
 long long ek_device_bytes(const ek_ctx* ctx);
 
