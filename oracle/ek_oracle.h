@@ -84,6 +84,26 @@ void eko_godunov_unsplit_impl(const eko_params* p, double* Uin, double* Uout, do
 int eko_run(const eko_params* p, double* U, int nStepmax, double tEnd, double* t_io, int* it_io,
             double* dt_last);
 
+/* ---- 2D MHD (SolverMHDMuscl<2>, impl 0), in ek_oracle_mhd.c; nbvar = 8, gw = 3 ---- */
+void eko_mhd_constoprim(const eko_params* p, const double* u, const double* mag, double* c, double* q);
+void eko_mhd_speed_info(const eko_params* p, const double* q, double* out);
+void eko_riemann_mhd(const eko_params* p, double* ql, double* qr, double* flux);
+double eko_compute_emfz(const eko_params* p, double qEdge[4][8]);
+void eko_mhd_trace_2d(const eko_params* p, const double qNb[3][3][8], const double bfNb[4][4][3],
+                      double dtdx, double dtdy, double qm[2][8], double qp[2][8], double qEdge[4][8]);
+void eko_mhd_init_orszag_tang(const eko_params* p, double* U);
+void eko_mhd_convert_to_primitives(const eko_params* p, const double* U, double* Q);
+double eko_mhd_inv_dt(const eko_params* p, const double* Q);
+void eko_mhd_compute_trace(const eko_params* p, const double* U, const double* Q, double* const out[8],
+                           double dtdx, double dtdy);
+void eko_mhd_compute_fluxes(const eko_params* p, double* const tr[8], double* Fx, double* Fy);
+void eko_mhd_compute_emf(const eko_params* p, double* const tr[8], double* Emf);
+void eko_mhd_update(const eko_params* p, double* U, const double* Fx, const double* Fy, const double* Emf,
+                    double dtdx, double dtdy);
+/* MHD time loop; dt is computed inside each step from the primitives of the current state. */
+int eko_mhd_run(const eko_params* p, double* U, int nStepmax, double tEnd, double* t_io, int* it_io,
+                double* dt_last);
+
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int eko_num_threads(void);
 

@@ -74,6 +74,19 @@ def lib():
         L.eko_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
         L.eko_run.restype = C.c_int
         L.eko_num_threads.restype = C.c_int
+        A8 = C.POINTER(C.c_double * 8)
+        L.eko_mhd_constoprim.argtypes = [P(EkoParams), D, D, D, D]
+        L.eko_mhd_speed_info.argtypes = [P(EkoParams), D, D]
+        L.eko_riemann_mhd.argtypes = [P(EkoParams), D, D, D]
+        L.eko_compute_emfz.argtypes = [P(EkoParams), A8]
+        L.eko_compute_emfz.restype = C.c_double
+        L.eko_mhd_trace_2d.argtypes = [P(EkoParams), D, D, C.c_double, C.c_double, D, D, D]
+        L.eko_mhd_init_orszag_tang.argtypes = [P(EkoParams), D]
+        L.eko_mhd_convert_to_primitives.argtypes = [P(EkoParams), D, D]
+        L.eko_mhd_inv_dt.argtypes = [P(EkoParams), D]
+        L.eko_mhd_inv_dt.restype = C.c_double
+        L.eko_mhd_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
+        L.eko_mhd_run.restype = C.c_int
         _L = L
     return _L
 
@@ -136,3 +149,75 @@ def interior(p: EkoParams, U: np.ndarray) -> np.ndarray:
     if p.dim == 2:
         return U[g:-g, g:-g, :]
     return U[g:-g, g:-g, g:-g, :]
+
+
+# ---------------------------------------------------------------- 2D MHD
+
+
+def orszag_tang(p: EkoParams) -> np.ndarray:
+    """InitOrszagTangFunctor2D (src/muscl/MHDInitFunctors2D.h:345-479) on the reference layout."""
+    U = np.zeros(ref_shape(p))
+    lib().eko_mhd_init_orszag_tang(C.byref(p), dp(U))
+    return U
+
+
+def mhd_run(p: EkoParams, U: np.ndarray, nStepmax: int, tEnd: float, t: float = 0.0, it: int = 0):
+    """SolverMHDMuscl<2> time loop; U is advanced in place. Returns (t, it, last dt)."""
+    tt, ii, dd = C.c_double(t), C.c_int(it), C.c_double()
+    rc = lib().eko_mhd_run(C.byref(p), dp(U), nStepmax, tEnd, C.byref(tt), C.byref(ii),
+                           C.byref(dd))
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return tt.value, ii.value, dd.value
+
+
+def mhd_primitives(p: EkoParams, U: np.ndarray) -> np.ndarray:
+    Q = np.zeros_like(U)
+    lib().eko_mhd_convert_to_primitives(C.byref(p), dp(U), dp(Q))
+    return Q
+
+
+def mhd_inv_dt(p: EkoParams, U: np.ndarray) -> float:
+    """Max over the interior of the fast speeds of the primitives of U (ghosts as given)."""
+    return lib().eko_mhd_inv_dt(C.byref(p), dp(mhd_primitives(p, U)))
+
+
+def mhd_constoprim(p: EkoParams, u, mag):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    mag = np.ascontiguousarray(mag, dtype=np.float64)
+    q = np.zeros(8)
+    c = C.c_double()
+    lib().eko_mhd_constoprim(C.byref(p), dp(u), dp(mag), C.byref(c), dp(q))
+    return q, c.value
+
+
+def mhd_speed_info(p: EkoParams, q) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.zeros(3)
+    lib().eko_mhd_speed_info(C.byref(p), dp(q), dp(out))
+    return out
+
+
+def riemann_mhd(p: EkoParams, ql, qr) -> np.ndarray:
+    a = np.array(ql, dtype=np.float64)
+    b = np.array(qr, dtype=np.float64)
+    f = np.zeros(8)
+    lib().eko_riemann_mhd(C.byref(p), dp(a), dp(b), dp(f))
+    return f
+
+
+def compute_emfz(p: EkoParams, qedge) -> float:
+    """compute_emf<EMFZ>; qedge rows are the RT, RB, LT, LB edge states."""
+    q = np.ascontiguousarray(qedge, dtype=np.float64).reshape(4, 8)
+    return lib().eko_compute_emfz(C.byref(p), q.ctypes.data_as(C.POINTER(C.c_double * 8)))
+
+
+def mhd_trace_2d(p: EkoParams, qnb, bfnb, dtdx: float, dtdy: float):
+    """trace_unsplit_mhd_2d: qnb [3][3][8], bfnb [4][4][3] -> (qm[2][8], qp[2][8], qEdge[4][8])."""
+    qnb = np.ascontiguousarray(qnb, dtype=np.float64).reshape(3, 3, 8)
+    bfnb = np.ascontiguousarray(bfnb, dtype=np.float64).reshape(4, 4, 3)
+    qm = np.zeros((2, 8))
+    qp = np.zeros((2, 8))
+    qe = np.zeros((4, 8))
+    lib().eko_mhd_trace_2d(C.byref(p), dp(qnb), dp(bfnb), dtdx, dtdy, dp(qm), dp(qp), dp(qe))
+    return qm, qp, qe

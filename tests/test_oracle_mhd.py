@@ -1,0 +1,366 @@
+"""Pins of the 2D MHD oracle (SolverMHDMuscl<2>, impl 0) against what mathematics fixes.
+
+The reference ships no golden vectors for the MHD step either, so every MHD function of
+the oracle is pinned here by a closed form, a limit that reduces to an already pinned
+solver, a symmetry of the equations, a conservation law or an exact solution, chosen so
+that a plausible transcription slip (dropped term, wrong sign, swapped corner or index)
+fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from euler_kokkos_amd.params import make_params
+
+ID, IP, IU, IV, IW, IA, IB, IC = range(8)
+
+
+def P(oracle, nx=16, ny=16, **kw):
+    kw.setdefault("gamma0", "1.666")
+    kw.setdefault("riemann", "hlld")
+    kw.setdefault("slope_type", "2")
+    kw.setdefault("smallr", "1e-7")
+    kw.setdefault("smallc", "1e-7")
+    return oracle.from_ek(make_params("MHD_Muscl_2D", nx, ny, **kw))
+
+
+def rand_state(rng, bscale=1.0, vscale=0.5):
+    return np.array([rng.uniform(0.5, 2.0), rng.uniform(0.3, 2.0), *rng.uniform(-vscale, vscale, 3),
+                     *rng.uniform(-bscale, bscale, 3)])
+
+
+def mhd_physical_flux(q, g):
+    """Ideal-MHD x-flux of a primitive state (rho, p, u, v, w, Bx, By, Bz)."""
+    r, p, u, v, w, a, b, c = q
+    pt = p + 0.5 * (a * a + b * b + c * c)
+    E = p / (g - 1) + 0.5 * r * (u * u + v * v + w * w) + 0.5 * (a * a + b * b + c * c)
+    vb = u * a + v * b + w * c
+    return np.array([r * u, (E + pt) * u - a * vb, r * u * u + pt - a * a, r * u * v - a * b,
+                     r * u * w - a * c, 0.0, b * u - a * v, c * u - a * w])
+
+
+# ------------------------------------------------------------------ primitives, speeds
+
+def test_constoprim_closed_form(oracle):
+    p = P(oracle)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        u = np.array([rng.uniform(0.5, 2), 0, *rng.uniform(-1, 1, 3), *rng.uniform(-1, 1, 3)])
+        mag = rng.uniform(-1, 1, 3)
+        Bc = 0.5 * (u[IA:IC + 1] + mag)
+        vel = u[IU:IW + 1] / u[ID]
+        pr = rng.uniform(0.2, 2.0)
+        u[IP] = pr / (p.gamma0 - 1) + 0.5 * u[ID] * vel @ vel + 0.5 * Bc @ Bc
+        q, c = oracle.mhd_constoprim(p, u, mag)
+        np.testing.assert_allclose(q[IU:IW + 1], vel, rtol=1e-15)
+        np.testing.assert_allclose(q[IA:IC + 1], Bc, rtol=1e-15)
+        np.testing.assert_allclose(q[IP], pr, rtol=1e-12)
+        np.testing.assert_allclose(c, math.sqrt(p.gamma0 * pr / u[ID]), rtol=1e-12)
+    # density and pressure floors
+    q, _ = oracle.mhd_constoprim(p, [0.0, 0.0, 0, 0, 0, 1.0, 0, 0], [1.0, 0, 0])
+    assert q[ID] == p.smallr
+    assert q[IP] == q[ID] * p.smallp
+
+
+def test_fast_speed_limits(oracle):
+    p = P(oracle)
+    g = p.gamma0
+    r, pr, u, v = 1.3, 0.7, -0.4, 0.25
+    c = math.sqrt(g * pr / r)
+    s = oracle.mhd_speed_info(p, [r, pr, u, v, 0, 0, 0, 0])
+    np.testing.assert_allclose(s[:2], [c + abs(u), c + abs(v)], rtol=1e-15)
+    # field along x: the fast speed along the field is max(c, vA), across it sqrt(c^2 + vA^2)
+    for a in (0.3, 2.0):
+        va = a / math.sqrt(r)
+        s = oracle.mhd_speed_info(p, [r, pr, u, v, 0, a, 0, 0])
+        np.testing.assert_allclose(s[0], max(c, va) + abs(u), rtol=1e-13)
+        np.testing.assert_allclose(s[1], math.sqrt(c * c + va * va) + abs(v), rtol=1e-13)
+
+
+# ------------------------------------------------------------------ 1D Riemann solvers
+
+@pytest.mark.parametrize("rs", ["hlld", "hll", "llf"])
+def test_mhd_riemann_consistency(oracle, rs):
+    p = P(oracle, riemann=rs)
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        q = rand_state(rng)
+        f = oracle.riemann_mhd(p, q, q)
+        np.testing.assert_allclose(f, mhd_physical_flux(q, p.gamma0), rtol=1e-12, atol=1e-13)
+
+
+def test_hlld_without_field_is_hllc(oracle):
+    # with B = 0 the HLLD fan collapses to the HLLC one (same wave-speed estimate);
+    # the hydro HLLC is pinned against the exact Sod solution
+    p = P(oracle)
+    ph = oracle.from_ek(make_params("Hydro_Muscl_3D", 8, 8, 8, gamma0="1.666", riemann="hllc",
+                                    smallr="1e-7", smallc="1e-7"))
+    rng = np.random.default_rng(3)
+    import ctypes as C
+    for _ in range(40):
+        ql, qr = rand_state(rng, 0.0, 1.5), rand_state(rng, 0.0, 1.5)
+        ql[IA:] = 0
+        qr[IA:] = 0
+        f = oracle.riemann_mhd(p, ql, qr)
+        hl, hr, fh = ql[:5].copy(), qr[:5].copy(), np.zeros(5)
+        oracle.lib().eko_riemann_hydro(C.byref(ph), oracle.dp(hl), oracle.dp(hr), oracle.dp(fh))
+        np.testing.assert_allclose(f[:5], fh, rtol=1e-12, atol=1e-14)
+        assert np.all(f[IA:] == 0)
+
+
+@pytest.mark.parametrize("rs", ["hlld", "hll", "llf"])
+def test_mhd_riemann_symmetries(oracle, rs):
+    p = P(oracle, riemann=rs)
+    rng = np.random.default_rng(4)
+    M = np.array([1, 1, -1, 1, 1, -1, 1, 1.0])  # x -> -x: u and Bx change sign
+    Bflip = np.array([1, 1, 1, 1, 1, -1, -1, -1.0])
+    for _ in range(20):
+        ql, qr = rand_state(rng), rand_state(rng)
+        f = oracle.riemann_mhd(p, ql, qr)
+        fm = oracle.riemann_mhd(p, M * qr, M * ql)
+        np.testing.assert_allclose(fm, -M * f, rtol=1e-11, atol=1e-13)
+        fb = oracle.riemann_mhd(p, Bflip * ql, Bflip * qr)
+        np.testing.assert_allclose(fb, f * np.array([1, 1, 1, 1, 1, 1, -1, -1.0]), rtol=1e-11,
+                                   atol=1e-13)
+
+
+def test_hlld_resolves_stationary_contact(oracle):
+    # a density jump at rest with continuous p, v and B is an exact HLLD solution: no mass flux
+    p = P(oracle)
+    ql = np.array([1.0, 1.0, 0.0, 0.2, -0.1, 0.5, 0.3, 0.1])
+    qr = ql.copy()
+    qr[ID] = 0.25
+    f = oracle.riemann_mhd(p, ql, qr)
+    assert abs(f[ID]) < 1e-15
+    np.testing.assert_allclose(f[IU], mhd_physical_flux(ql, p.gamma0)[IU], rtol=1e-14)
+    fh = oracle.riemann_mhd(P(oracle, riemann="hll"), ql, qr)
+    assert abs(fh[ID]) > 1e-3  # HLL smears it (shows the test has teeth)
+
+
+# ------------------------------------------------------------------ 2D EMF solver
+
+def test_emfz_uniform_state_is_v_cross_b(oracle):
+    p = P(oracle)
+    rng = np.random.default_rng(5)
+    for vs in (0.3, 6.0):  # subsonic (central branch) and supersonic (upwind corner)
+        for _ in range(10):
+            q = rand_state(rng, 1.0, vs)
+            E = oracle.compute_emfz(p, np.tile(q, (4, 1)))
+            np.testing.assert_allclose(E, q[IU] * q[IB] - q[IV] * q[IA], rtol=1e-12, atol=1e-14)
+
+
+def test_emfz_field_reversal_and_rotation(oracle):
+    p = P(oracle)
+    rng = np.random.default_rng(6)
+    Bflip = np.array([1, 1, 1, 1, 1, -1, -1, -1.0])
+
+    def rot(q):  # frame rotated by -90 deg: x' = y, y' = -x
+        r = q.copy()
+        r[IU], r[IV] = q[IV], -q[IU]
+        r[IA], r[IB] = q[IB], -q[IA]
+        return r
+
+    for _ in range(20):
+        qe = np.array([rand_state(rng) for _ in range(4)])  # RT, RB, LT, LB
+        E = oracle.compute_emfz(p, qe)
+        np.testing.assert_allclose(oracle.compute_emfz(p, qe * Bflip), -E, rtol=1e-11, atol=1e-13)
+        RT, RB, LT, LB = qe
+        # cells around the corner move (-,-)->(-,+), (-,+)->(+,+), (+,-)->(-,-), (+,+)->(+,-)
+        qr = np.array([rot(LT), rot(RT), rot(LB), rot(RB)])
+        np.testing.assert_allclose(oracle.compute_emfz(p, qr), E, rtol=1e-10, atol=1e-13)
+
+
+# ------------------------------------------------------------------ trace
+
+def test_trace_uniform_state(oracle):
+    p = P(oracle)
+    q = np.array([1.2, 0.8, 0.3, -0.2, 0.1, 0.4, -0.6, 0.25])
+    qnb = np.tile(q, (3, 3, 1))
+    bfnb = np.tile(q[IA:IC + 1], (4, 4, 1))
+    qm, qp, qe = oracle.mhd_trace_2d(p, qnb, bfnb, 0.3, 0.2)
+    for s in (*qm, *qp, *qe):
+        np.testing.assert_allclose(s, q, rtol=1e-15, atol=1e-16)
+
+
+def test_trace_linear_data_zero_dt(oracle):
+    # dt = 0: states are the MC-limited linear reconstruction (exact for linear data)
+    p = P(oracle)
+    gx = np.array([0.01, 0.02, 0.03, -0.02, 0.01, 0, 0.05, 0.04])
+    gy = np.array([-0.02, 0.01, 0.02, 0.03, -0.01, 0.06, 0, -0.03])
+    q0 = np.array([1.0, 1.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6])
+    qnb = np.array([[q0 + (di - 1) * gx + (dj - 1) * gy for dj in range(3)] for di in range(3)])
+    # face fields: Bx on x faces varies along y only, By on y faces along x only (div-free)
+    bfnb = np.zeros((4, 4, 3))
+    for di in range(4):
+        for dj in range(4):
+            bfnb[di, dj] = [0.4 + (dj - 1) * gy[IA], 0.5 + (di - 1) * gx[IB], 0.6]
+    qm, qp, qe = oracle.mhd_trace_2d(p, qnb, bfnb, 0.0, 0.0)
+    hyd = [ID, IP, IU, IV, IW, IC]
+    np.testing.assert_allclose(qm[0][hyd], (q0 + gx / 2)[hyd], rtol=1e-14)
+    np.testing.assert_allclose(qp[0][hyd], (q0 - gx / 2)[hyd], rtol=1e-14)
+    np.testing.assert_allclose(qm[1][hyd], (q0 + gy / 2)[hyd], rtol=1e-14)
+    np.testing.assert_allclose(qp[1][hyd], (q0 - gy / 2)[hyd], rtol=1e-14)
+    assert qm[0][IA] == bfnb[2, 1, 0] and qp[0][IA] == bfnb[1, 1, 0]
+    assert qm[1][IB] == bfnb[1, 2, 1] and qp[1][IB] == bfnb[1, 1, 1]
+    RT, RB, LT, LB = qe
+    for s, sx, sy in ((RT, 1, 1), (RB, 1, -1), (LT, -1, 1), (LB, -1, -1)):
+        np.testing.assert_allclose(s[[ID, IP, IU, IV, IC]],
+                                   (q0 + sx * gx / 2 + sy * gy / 2)[[ID, IP, IU, IV, IC]], rtol=1e-14)
+        # edge value of Bx: face value shifted half a cell along y
+        np.testing.assert_allclose(s[IA], 0.4 + sy * gy[IA] / 2, rtol=1e-14)
+        np.testing.assert_allclose(s[IB], 0.5 + sx * gx[IB] / 2, rtol=1e-14)
+
+
+def test_trace_advection_predictor(oracle):
+    # uniform velocity and pressure, no field: density is advected by half a step
+    p = P(oracle)
+    u, v = 0.5, -0.25
+    gx, gy = 0.0625, 0.125
+    qnb = np.zeros((3, 3, 8))
+    for di in range(3):
+        for dj in range(3):
+            qnb[di, dj] = [1 + (di - 1) * gx + (dj - 1) * gy, 1.0, u, v, 0, 0, 0, 0]
+    dtdx, dtdy = 0.5, 0.25
+    qm, qp, qe = oracle.mhd_trace_2d(p, qnb, np.zeros((4, 4, 3)), dtdx, dtdy)
+    r = 1.0 - u * gx / 2 * dtdx - v * gy / 2 * dtdy
+    assert qm[0][ID] == r + gx / 2 and qp[0][ID] == r - gx / 2
+    assert qm[1][ID] == r + gy / 2 and qp[1][ID] == r - gy / 2
+    assert qe[0][ID] == r + (gx / 2 + gy / 2)
+    for s in (*qm, *qp, *qe):
+        assert s[IU] == u and s[IV] == v and s[IP] == 1.0
+
+
+# ------------------------------------------------------------------ whole steps
+
+def orszag_tang(oracle, n, **kw):
+    kw.setdefault("bc", (3,) * 6)
+    p = P(oracle, n, n, cfl="0.8", **kw)
+    return p, oracle.orszag_tang(p)
+
+
+def test_orszag_tang_initial_condition(oracle):
+    p, U = orszag_tang(oracle, 16)
+    g = p.gamma0
+    B0 = 1 / math.sqrt(4 * math.pi)
+    d0 = g * g / (4 * math.pi)
+    for i, j in ((0, 0), (5, 9), (20, 3)):
+        x = p.xmin + p.dx / 2 + (i - 3) * p.dx
+        y = p.ymin + p.dy / 2 + (j - 3) * p.dy
+        np.testing.assert_allclose(U[i, j, [ID, IU, IV, IA, IB]],
+                                   [d0, -d0 * math.sin(2 * math.pi * y), d0 * math.sin(2 * math.pi * x),
+                                    -B0 * math.sin(2 * math.pi * y), B0 * math.sin(4 * math.pi * x)],
+                                   rtol=1e-14, atol=1e-16)
+        Bx = 0.5 * (U[i, j, IA] + U[i + 1, j, IA])
+        By = 0.5 * (U[i, j, IB] + U[i, j + 1, IB])
+        E = g / (4 * math.pi) / (g - 1) + 0.5 * (U[i, j, IU] ** 2 + U[i, j, IV] ** 2) / d0 \
+            + 0.5 * (Bx * Bx + By * By)
+        np.testing.assert_allclose(U[i, j, IP], E, rtol=1e-14)
+
+
+def divb(p, U):
+    g = p.gw
+    A, B = U[..., IA], U[..., IB]
+    d = (A[g + 1:-g + 1, g:-g] - A[g:-g, g:-g]) / p.dx + (B[g:-g, g + 1:-g + 1] - B[g:-g, g:-g]) / p.dy
+    return np.abs(d).max()
+
+
+def test_orszag_tang_conservation_and_divb(oracle):
+    p, U = orszag_tang(oracle, 48)
+    oracle.lib().eko_make_boundaries(__import__("ctypes").byref(p), oracle.dp(U))
+    assert divb(p, U) < 1e-13
+    I0 = oracle.interior(p, U).sum(axis=(0, 1))
+    t, it, _ = oracle.mhd_run(p, U, 60, 10.0)
+    assert it == 60
+    oracle.lib().eko_make_boundaries(__import__("ctypes").byref(p), oracle.dp(U))
+    I1 = oracle.interior(p, U).sum(axis=(0, 1))
+    # mass, momenta, energy, face fields: telescoping sums on a periodic grid
+    np.testing.assert_allclose(I1[[ID, IP, IU, IV, IA, IB]], I0[[ID, IP, IU, IV, IA, IB]],
+                               rtol=1e-12, atol=1e-12)
+    bmax = np.abs(U[..., IA:IC]).max()
+    assert divb(p, U) < 1e-12 * bmax / p.dx
+    assert np.isfinite(U).all()
+
+
+def test_at_rest_uniform_field_is_bitwise_steady(oracle):
+    for bc in ((3,) * 6, (1, 2, 2, 1, 3, 3)):
+        p = P(oracle, 12, 10, bc=bc)
+        U = np.zeros(oracle.ref_shape(p))
+        U[..., ID] = 1.1
+        U[..., IA], U[..., IB], U[..., IC] = 0.3, -0.4, 0.2
+        U[..., IP] = 0.9 / (p.gamma0 - 1) + 0.5 * (0.09 + 0.16 + 0.04)
+        g = p.gw
+        ref = U[g:-g, g:-g].copy()
+        if bc[0] == 1:  # reflecting walls flip the normal field: use a tangential-only field
+            U[..., IA] = 0.0
+            U[..., IB] = 0.0
+            U[..., IP] = 0.9 / (p.gamma0 - 1) + 0.5 * 0.04
+            ref = U[g:-g, g:-g].copy()
+        oracle.mhd_run(p, U, 12, 1e9)
+        assert np.array_equal(U[g:-g, g:-g], ref)
+
+
+def transpose_state(U):
+    T = U.transpose(1, 0, 2).copy()
+    T[..., [IU, IV]] = T[..., [IV, IU]]
+    T[..., [IA, IB]] = T[..., [IB, IA]]
+    return T
+
+
+def test_transpose_symmetry(oracle):
+    # swapping x and y (with u<->v, Bx<->By) commutes with the step: pins the y-direction
+    # code paths (rotated y fluxes, y slopes, EMF corners) against the x ones
+    p, U = orszag_tang(oracle, 32, bc=(3, 3, 3, 3, 3, 3))
+    oracle.mhd_run(p, U, 5, 1e9)  # a generic, non-symmetric state
+    V = transpose_state(U)
+    oracle.mhd_run(p, U, 10, 1e9)
+    oracle.mhd_run(p, V, 10, 1e9)
+    g = p.gw
+    np.testing.assert_allclose(transpose_state(V)[g:-g, g:-g], U[g:-g, g:-g], rtol=1e-10, atol=1e-12)
+
+
+def alfven_wave(oracle, nx, amp=1e-4):
+    """Small-amplitude Alfven wave along x (Bx = 1, rho = 1: travels at speed 1, unit period).
+
+    Linearly polarized in y: the 2D primitives take Bz as half the stored value (the
+    reference passes a zero neighbour for the z face, src/muscl/MHDRunFunctors2D.h:137),
+    so a circularly polarized wave would not be a solution of the 2D step. The
+    amplitude keeps the O(amp^2) compressive coupling far below the truncation error."""
+    p = P(oracle, nx, 4, ymax=str(4.0 / nx), cfl="0.8", bc=(3,) * 6)
+    U = np.zeros(oracle.ref_shape(p))
+    x = p.xmin + p.dx / 2 + (np.arange(p.isize) - p.gw) * p.dx
+    by = amp * np.sin(2 * np.pi * x)
+    U[..., ID] = 1.0
+    U[..., IA] = 1.0
+    U[..., IB] = by[:, None]
+    U[..., IV] = -by[:, None]
+    U[..., IP] = 0.1 / (p.gamma0 - 1) + 0.5 * (by * by)[:, None] * 2 + 0.5
+    return p, U
+
+
+def test_alfven_wave_second_order(oracle):
+    errs = []
+    for nx in (32, 64):
+        p, U = alfven_wave(oracle, nx)
+        U0 = U.copy()
+        t, _, _ = oracle.mhd_run(p, U, 100000, 1.0)
+        assert abs(t - 1.0) < 1e-12
+        g = p.gw
+        errs.append(np.abs(U[g:-g, g:-g, IB] - U0[g:-g, g:-g, IB]).mean() / 1e-4)
+    assert errs[1] < 0.05
+    assert errs[0] / errs[1] > 3.0, errs  # second order: the error drops ~4x per halving
+
+
+def test_mhd_sod_without_field_matches_exact(oracle):
+    from test_oracle import exact_sod
+    nx = 200
+    p = P(oracle, nx, 4, gamma0="1.4", cfl="0.5", bc=(2, 2, 3, 3, 3, 3))
+    U = np.zeros(oracle.ref_shape(p))
+    x = p.xmin + p.dx / 2 + (np.arange(p.isize) - p.gw) * p.dx
+    left = x < 0.5
+    U[..., ID] = np.where(left, 1.0, 0.125)[:, None]
+    U[..., IP] = (np.where(left, 1.0, 0.1) / (p.gamma0 - 1))[:, None]
+    t, _, _ = oracle.mhd_run(p, U, 100000, 0.2)
+    I = oracle.interior(p, U)[:, 1, :]
+    ex = exact_sod(x[3:-3] - 0.5, t, p.gamma0)
+    assert np.abs(I[:, ID] - ex[:, 0]).mean() < 0.012
