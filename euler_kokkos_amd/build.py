@@ -40,6 +40,8 @@ UNITS = [
     ("ek_fp64.hip", ["-ffp-contract=off"]),
     ("ek_fp32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("ek_capi.hip", ["-ffp-contract=off"]),
+    ("ek_mhd_fp64.hip", ["-ffp-contract=off"]),
+    ("ek_mhd_fp32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
 ]
 HOST_UNITS = ["ConfigMap.cpp", "HydroParams.cpp", "SolverBase.cpp", "SolverHydroMuscl.cpp",
               "SolverFactory.cpp", "Problems.cpp", "IO_VTK.cpp", "ek_host_capi.cpp"]

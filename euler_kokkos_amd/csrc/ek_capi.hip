@@ -104,20 +104,44 @@ static void fill_geom(const ek_params& p, ek::Geom<T>& g, ek::Gas<T>& gas)
   gas.entho = (T)(1.0 / (p.gamma0 - 1.0));
 }
 
+static int validate_common(const ek_params* p, std::string& why)
+{
+  if (p->precision != EK_FP64 && p->precision != EK_FP32)
+    return why = "precision must be EK_FP64 or EK_FP32", 1;
+  for (int i = 0; i < 2 * p->dim; i++)
+    if (p->bc[i] < 1 || p->bc[i] > 3)
+      return why = "boundary types must be 1 (reflecting), 2 (outflow) or 3 (periodic)", 1;
+  if (!(p->dx > 0) || !(p->dy > 0) || (p->dim == 3 && !(p->dz > 0)))
+    return why = "cell sizes must be positive", 1;
+  if (p->nx < p->gw || p->ny < p->gw || (p->dim == 3 && p->nz < p->gw))
+    return why = "every axis needs at least ghostWidth interior cells", 1;
+  return 0;
+}
+
 static int validate(const ek_params* p, std::string& why)
 {
   if (p->dim != 2 && p->dim != 3)
     return why = "dim must be 2 or 3", 1;
-  if (p->mhd)
-    return why = "MHD solvers are not provided by this library build", 1;
+  if (p->mhd) {
+    // SolverMHDMuscl<2>, impl 0 (src/muscl/SolverMHDMuscl.cpp:369-454)
+    if (p->dim != 2)
+      return why = "MHD is provided in 2D (MHD_Muscl_2D) by this library build", 1;
+    if (p->gw != 3)
+      return why = "MHD MUSCL needs ghostWidth 3", 1;
+    if (p->nbvar != 8)
+      return why = "nbvar must be 8 for MHD", 1;
+    if (p->riemann != ek::RS_HLLD && p->riemann != ek::RS_HLL && p->riemann != ek::RS_LLF)
+      return why = "MHD riemann solver must be hlld, hll or llf (riemann_mhd dispatch)", 1;
+    if (!(p->slope_type == 1.0 || p->slope_type == 2.0))
+      return why = "the MHD trace supports slope_type 1 and 2 only", 1;
+    if (p->cIso > 0)
+      return why = "isothermal MHD (cIso > 0) is not provided", 1;
+    return validate_common(p, why);
+  }
   if (p->gw != 2)
     return why = "hydro MUSCL needs ghostWidth 2", 1;
   if (p->nbvar != (p->dim == 2 ? 4 : 5))
     return why = "nbvar must be 4 (2D) or 5 (3D) for hydro", 1;
-  if (p->nx < p->gw || p->ny < p->gw || (p->dim == 3 && p->nz < p->gw))
-    return why = "every axis needs at least ghostWidth interior cells", 1;
-  if (p->precision != EK_FP64 && p->precision != EK_FP32)
-    return why = "precision must be EK_FP64 or EK_FP32", 1;
   if (p->riemann != ek::RS_APPROX && p->riemann != ek::RS_LLF && p->riemann != ek::RS_HLLC &&
       !(p->riemann == ek::RS_HLL && p->dim == 2))
     return why = "riemann solver not available for hydro in this dimension (3D has no HLL in the "
@@ -125,12 +149,7 @@ static int validate(const ek_params* p, std::string& why)
            1;
   if (!(p->slope_type == 0.0 || p->slope_type == 1.0 || p->slope_type == 2.0))
     return why = "slope_type must be 0, 1 or 2", 1;
-  for (int i = 0; i < 2 * p->dim; i++)
-    if (p->bc[i] < 1 || p->bc[i] > 3)
-      return why = "boundary types must be 1 (reflecting), 2 (outflow) or 3 (periodic)", 1;
-  if (!(p->dx > 0) || !(p->dy > 0) || (p->dim == 3 && !(p->dz > 0)))
-    return why = "cell sizes must be positive", 1;
-  return 0;
+  return validate_common(p, why);
 }
 
 extern "C" int ek_create(const ek_params* params, ek_ctx** out)
@@ -299,6 +318,11 @@ extern "C" int ek_download(ek_ctx* c, double* host, int layout)
 
 static hipError_t launch_inv_dt_into(ek_ctx* c, unsigned long long* dst)
 {
+  if (c->p.mhd) {
+    if (c->p.precision == EK_FP64)
+      return ek::fp64::mhd_inv_dt((const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
+    return ek::fp32::mhd_inv_dt((const float*)c->U[c->cur], c->g32, c->gas32, dst, c->stream);
+  }
   if (c->p.precision == EK_FP64)
     return ek::fp64::inv_dt(c->p.dim, (const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
   return ek::fp32::inv_dt(c->p.dim, (const float*)c->U[c->cur], c->g32, c->gas32, dst, c->stream);
@@ -320,6 +344,12 @@ extern "C" int ek_inv_dt(ek_ctx* c, double* inv_dt)
 static hipError_t launch_one(ek_ctx* c, const ek::StepCtl& ctl, void* in, void* out)
 {
   const ek_params& p = c->p;
+  if (p.mhd) {
+    if (p.precision == EK_FP64)
+      return ek::fp64::mhd_step(p.riemann, c->st, (const double*)in, (double*)out, c->g64, c->gas64, ctl,
+                                c->stream);
+    return ek::fp32::mhd_step(p.riemann, c->st, (const float*)in, (float*)out, c->g32, c->gas32, ctl, c->stream);
+  }
   if (p.precision == EK_FP64)
     return ek::fp64::step(p.dim, p.riemann, c->st, (const double*)in, (double*)out, c->g64, c->gas64, ctl,
                           c->cz, c->stream);

@@ -1,5 +1,6 @@
 // ek_internal.h -- per-precision launch entry points, implemented in ek_fp64.hip
-// (bit-exact FP64) and ek_fp32.hip (FP32 bulk), called by the C ABI in ek_capi.hip.
+// (bit-exact FP64) and ek_fp32.hip (FP32 bulk), the 2D MHD ones in ek_mhd_fp64.hip /
+// ek_mhd_fp32.hip; called by the C ABI in ek_capi.hip.
 #pragma once
 
 #include "ek_hydro.h"
@@ -16,6 +17,11 @@ hipError_t ref_to_soa(int dim, const double* R, double* U, const Geom<double>& g
                       int isize, int jsize, int ksize, hipStream_t s);
 hipError_t soa_to_ref(int dim, const double* U, double* R, const Geom<double>& geo, int gw, int nbvar,
                       int isize, int jsize, int ksize, int mhd, hipStream_t s);
+// 2D MHD (ek_mhd_fp64.hip)
+hipError_t mhd_step(int rs, int st, const double* Uin, double* Uout, const Geom<double>& geo, const Gas<double>& gas,
+                    const StepCtl& ctl, hipStream_t s);
+hipError_t mhd_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
+                      hipStream_t s);
 } // namespace fp64
 namespace fp32 {
 hipError_t step(int dim, int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo,
@@ -28,5 +34,10 @@ hipError_t ref_to_soa(int dim, const double* R, float* U, const Geom<float>& geo
                       int isize, int jsize, int ksize, hipStream_t s);
 hipError_t soa_to_ref(int dim, const float* U, double* R, const Geom<float>& geo, int gw, int nbvar,
                       int isize, int jsize, int ksize, int mhd, hipStream_t s);
+// 2D MHD (ek_mhd_fp32.hip)
+hipError_t mhd_step(int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo, const Gas<float>& gas,
+                    const StepCtl& ctl, hipStream_t s);
+hipError_t mhd_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
+                      hipStream_t s);
 } // namespace fp32
 } // namespace ek

@@ -72,6 +72,40 @@ def implode(p, density_outer="1.0", pressure_outer="1.0", density_inner="0.125",
     return U
 
 
+def orszag_tang(p) -> np.ndarray:
+    """InitOrszagTangFunctor2D (src/muscl/MHDInitFunctors2D.h:345-479), 2D MHD.
+
+    Evaluated on the full ghost-extended grid exactly as the functor does (the energy pass
+    reads the face fields of the right and top neighbours, ghosts included), with libm's
+    sin through ``math.sin``; the interior is returned in the device SoA layout."""
+    import math
+    g = p.gw
+    isz, jsz = p.nx + 2 * g, p.ny + 2 * g
+    twopi = 2 * math.pi
+    gamma0 = p.gamma0
+    B0 = 1.0 / math.sqrt(2 * twopi)
+    p0 = gamma0 / (2 * twopi)
+    d0 = gamma0 * p0
+    xs = [p.xmin + p.dx / 2 + (i - g) * p.dx for i in range(isz)]
+    ys = [p.ymin + p.dy / 2 + (j - g) * p.dy for j in range(jsz)]
+    sy = np.array([math.sin(y * twopi) for y in ys])
+    sx = np.array([math.sin(x * twopi) for x in xs])
+    s2x = np.array([math.sin(2.0 * x * twopi) for x in xs])
+    U = np.zeros((8, jsz, isz))
+    U[0] = d0
+    U[2] = (-d0 * 1.0 * sy)[:, None]
+    U[3] = (d0 * 1.0 * sx)[None, :]
+    U[5] = (-B0 * sy)[:, None]
+    U[6] = (B0 * s2x)[None, :]
+    two_pi_e = 4.0 * math.asin(1.0)
+    p0e = gamma0 / (2.0 * two_pi_e)
+    d, u, v, a, b = U[0, :-1, :-1], U[2, :-1, :-1], U[3, :-1, :-1], U[5], U[6]
+    bx = (a[:-1, :-1] + a[:-1, 1:]) / 2
+    by = (b[:-1, :-1] + b[1:, :-1]) / 2
+    U[1, :-1, :-1] = p0e / (gamma0 - 1.0) + 0.5 * ((u * u) / d + (v * v) / d) + 0.5 * (bx * bx + by * by)
+    return np.ascontiguousarray(U[:, g:-g, g:-g])
+
+
 def soa_from_reference(p, R: np.ndarray) -> np.ndarray:
     """Interior of a reference-layout array U(i,j[,k],var) as the device SoA U[var][k][j][i]."""
     g = p.gw

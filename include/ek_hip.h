@@ -1,8 +1,9 @@
 /*
  * ek_hip.h -- C ABI of the MI355X-native MUSCL-Hancock Godunov step (libek_hip.so).
  *
- * The library replaces the device side of euler_kokkos' hydro MUSCL solvers
- * (SolverHydroMuscl<2|3>, src/muscl/SolverHydroMuscl.{h,cpp}): the Kokkos functors of one
+ * The library replaces the device side of euler_kokkos' MUSCL solvers -- hydro
+ * (SolverHydroMuscl<2|3>, src/muscl/SolverHydroMuscl.{h,cpp}) and 2D MHD
+ * (SolverMHDMuscl<2>, src/muscl/SolverMHDMuscl.{h,cpp}): the Kokkos functors of one
  * impl-0 godunov step, the ghost fill and the CFL reduction. A host solver keeps the
  * reference's Solver/ConfigMap interface and calls these entry points (see the C++ host
  * in euler_kokkos_amd/host/). Paths cited are relative to the euler_kokkos source tree.
@@ -43,7 +44,7 @@ typedef struct {
   int nx, ny, nz;
   int gw;        /* ghostWidth: 2 (hydro), 3 (MHD) */
   int nbvar;     /* 4 (hydro 2D), 5 (hydro 3D), 8 (MHD) */
-  int mhd;       /* 0 hydro */
+  int mhd;       /* 0 hydro, 1 MHD (2D: HLLD/HLL/LLF fluxes + constrained transport) */
   int riemann;   /* 0 approx, 1 llf, 2 hll, 3 hllc, 4 hlld (src/shared/enums.h:142-149) */
   int bc[6];     /* xmin,xmax,ymin,ymax,zmin,zmax: 1 reflecting, 2 outflow, 3 periodic */
   int precision; /* EK_FP64 or EK_FP32 */
@@ -74,12 +75,16 @@ int ek_download(ek_ctx* ctx, double* host, int layout);
 
 /* CFL reduction on the current state: *inv_dt = max over interior cells of
  * (c+|u|)/dx + (c+|v|)/dy [+ (c+|w|)/dz]. Replaces ComputeDtFunctor2D/3D::apply
- * (src/muscl/HydroRunFunctors2D.h:18-82, HydroRunFunctors3D.h:16-87). Synchronizes. */
+ * (src/muscl/HydroRunFunctors2D.h:18-82, HydroRunFunctors3D.h:16-87). For MHD the speeds
+ * are the fast magnetosonic ones of the primitives (convertToPrimitives +
+ * ComputeDtFunctor2D_MHD, src/muscl/MHDRunFunctors2D.h:16-159). Synchronizes. */
 int ek_inv_dt(ek_ctx* ctx, double* inv_dt);
 
 /* One godunov_unsplit step with a host-chosen dt (SolverHydroMuscl::godunov_unsplit,
  * src/muscl/SolverHydroMuscl.h:686-700 and .cpp:214-390): ghost fill, primitives,
- * slopes/trace, Riemann fluxes and update, fused in one launch. Asynchronous. */
+ * slopes/trace, Riemann fluxes and update, fused in one launch. For MHD the launch is
+ * SolverMHDMuscl<2>::godunov_unsplit_impl (src/muscl/SolverMHDMuscl.cpp:369-454) after
+ * its compute_dt: trace, face fluxes, edge EMFs, flux and EMF updates. Asynchronous. */
 int ek_step(ek_ctx* ctx, double dt);
 
 /* Up to nsteps iterations of the reference time loop (src/main.cpp:126-131 with
