@@ -95,6 +95,68 @@ def cpu_baseline(n, seconds):
             "ref_style_mcell_per_s": steps * (n + 4) ** 3 / el * 1e-6}
 
 
+def mhd_c3(steps, with_cpu, cpu_seconds):
+    """2D ideal-MHD Orszag-Tang vortex, 1024^2, periodic, HLLD + constrained transport,
+    gamma 1.6666667 (float-parsed), CFL 0.8, MC slopes: FP64 and FP32 bulk on the GPU, the
+    discrete max |div B| of the FP64 result, and the oracle on the host cores."""
+    from euler_kokkos_amd import native, problems
+    from euler_kokkos_amd.params import make_params
+    n = 1024
+    kw = dict(gamma0="1.6666667", cfl="0.8", slope_type="2", riemann="hlld", smallr="1e-7",
+              smallc="1e-7", bc=(3,) * 6)
+    res = {"workload": f"2D MHD Orszag-Tang {n}^2 periodic, HLLD + CT, MC slopes, gamma 5/3, "
+                       f"CFL 0.8, {steps} steps", "grid": [n, n], "solver": "MHD_Muscl_2D"}
+    out = {}
+    for name, prec in (("fp64", native.EK_FP64), ("fp32", native.EK_FP32)):
+        ekp = make_params("MHD_Muscl_2D", n, n, precision=prec, **kw)
+        U0 = problems.orszag_tang(ekp)
+        ctx = native.Context(ekp)
+        try:
+            ctx.upload(U0, native.EK_LAYOUT_SOA)
+            b0 = ctx.budgets()
+            device_sync()
+            t0 = time.perf_counter()
+            t, it, dt = ctx.run(steps, steps, 1e30)
+            device_sync()
+            wall = time.perf_counter() - t0
+            ev_ms, launches = ctx.last_run_time()
+            b1 = ctx.budgets()
+            U = ctx.download(native.EK_LAYOUT_SOA)
+        finally:
+            ctx.close()
+        A, B = U[5], U[6]
+        divb = np.abs((np.roll(A, -1, axis=1) - A) / ekp.dx + (np.roll(B, -1, axis=0) - B) / ekp.dy).max()
+        out[name] = U
+        res[name] = {"value": steps * n * n / wall, "unit": "cell-updates/s",
+                     "ms_per_step": wall / steps * 1e3, "avg_launch_ms": ev_ms / max(launches, 1),
+                     "achieved_GBps": n * n * 2 * 8 * (8 if prec == native.EK_FP64 else 4)
+                     / (ev_ms / max(launches, 1) * 1e-3) / 1e9,
+                     "max_abs_divB": float(divb), "divB_definition":
+                     "max over cells |(Bx(i+1,j)-Bx(i,j))/dx + (By(i,j+1)-By(i,j))/dy| on face fields",
+                     "mass_drift_rel": float(abs(b1[0] - b0[0]) / abs(b0[0])),
+                     "energy_drift_rel": float(abs(b1[1] - b0[1]) / abs(b0[1]))}
+    d64, d32 = out["fp64"], out["fp32"]
+    res["fp32"]["l1_rho_rel"] = float(np.abs(d32[0] - d64[0]).mean() / np.abs(d64[0]).mean())
+    res["fp32"]["l1_E_rel"] = float(np.abs(d32[1] - d64[1]).mean() / np.abs(d64[1]).mean())
+    if with_cpu:
+        from oracle import pyoracle
+        ekp = make_params("MHD_Muscl_2D", n, n, **kw)
+        op = pyoracle.from_ek(ekp)
+        R = pyoracle.orszag_tang(op)
+        t0 = time.perf_counter()
+        pyoracle.mhd_run(op, R, 1, 1e30)
+        one = time.perf_counter() - t0
+        k = int(max(1, min(50, cpu_seconds // max(one, 1e-3))))
+        t0 = time.perf_counter()
+        pyoracle.mhd_run(op, R, k + 1, 1e30, 0.0, 1)
+        el = time.perf_counter() - t0
+        cores = pyoracle.lib().eko_num_threads()
+        res["cpu_baseline"] = {"value": k * n * n / el, "unit": "cell-updates/s", "cores": int(cores),
+                               "kind": "port", "sample": f"oracle MHD step on the same {n}^2 Orszag-Tang, "
+                               f"{k} steps after 1 untimed step, {cores} OpenMP threads"}
+    return res
+
+
 def copy_calibration(gib=1.0):
     """A device-to-device copy of known size, for scaling rocprofv3 FETCH/WRITE counters."""
     hip = C.CDLL("libamdhip64.so")
@@ -121,6 +183,8 @@ def main() -> int:
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 bulk run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-mhd", action="store_true", help="skip the 2D MHD (Orszag-Tang) line")
+    ap.add_argument("--mhd-steps", type=int, default=500)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build has no multi-GPU "
@@ -181,6 +245,8 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(n, a.cpu_seconds)
     else:
         out["cpu_baseline"] = None
+    if not a.no_mhd:
+        out["mhd2d"] = mhd_c3(a.mhd_steps, not a.no_cpu, a.cpu_seconds / 2)
     print(json.dumps(out))
     return 0
 

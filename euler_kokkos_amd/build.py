@@ -43,7 +43,7 @@ UNITS = [
     ("ek_mhd_fp64.hip", ["-ffp-contract=off"]),
     ("ek_mhd_fp32.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
 ]
-HOST_UNITS = ["ConfigMap.cpp", "HydroParams.cpp", "SolverBase.cpp", "SolverHydroMuscl.cpp",
+HOST_UNITS = ["ConfigMap.cpp", "HydroParams.cpp", "SolverBase.cpp", "SolverHydroMuscl.cpp", "SolverMHDMuscl.cpp",
               "SolverFactory.cpp", "Problems.cpp", "IO_VTK.cpp", "ek_host_capi.cpp"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
               "-I", os.path.join(ROOT, "include"), "-I", HOST]

@@ -1,6 +1,8 @@
 // Problems.cpp -- see Problems.h.
 #include "Problems.h"
 
+#include <cmath>
+
 namespace euler_kokkos {
 
 namespace {
@@ -105,6 +107,62 @@ void init_blast(const HydroParams& p, const ConfigMap& c, std::vector<double>& U
         U[s.at(ID, i, j, k)] = in ? din : dout;
         U[s.at(IP, i, j, k)] = in ? pin / (gamma0 - 1.0) : pout / (gamma0 - 1.0);
       }
+}
+
+// InitOrszagTangFunctor2D (src/muscl/MHDInitFunctors2D.h:345-479): both passes run over the
+// ghost-extended grid as in the reference (the energy pass reads the face fields of the
+// right and top neighbours, ghosts included); the interior is kept.
+void init_orszag_tang(const HydroParams& p, const ConfigMap& c, std::vector<double>& U)
+{
+  (void)c;
+  const int gw = p.ghostWidth;
+  const int isz = p.nx + 2 * gw, jsz = p.ny + 2 * gw;
+  const double TWOPI = 2 * 3.141592653589793238462643383279502884;
+  const double gamma0 = p.settings.gamma0;
+  const double B0 = 1.0 / std::sqrt(2 * TWOPI);
+  const double p0 = gamma0 / (2 * TWOPI);
+  const double d0 = gamma0 * p0;
+  const double v0 = 1.0;
+  std::vector<double> R((size_t)isz * jsz * 8, 0.0);
+  auto at = [&](int i, int j, int v) -> double& { return R[((size_t)i * jsz + j) * 8 + v]; };
+  for (int i = 0; i < isz; i++)
+    for (int j = 0; j < jsz; j++) {
+      const double xPos = p.xmin + p.dx / 2 + (i - gw) * p.dx;
+      const double yPos = p.ymin + p.dy / 2 + (j - gw) * p.dy;
+      at(i, j, ID) = d0;
+      at(i, j, IU) = -d0 * v0 * std::sin(yPos * TWOPI);
+      at(i, j, IV) = d0 * v0 * std::sin(xPos * TWOPI);
+      at(i, j, IW) = 0.0;
+      at(i, j, IA) = -B0 * std::sin(yPos * TWOPI);
+      at(i, j, IB) = B0 * std::sin(2.0 * xPos * TWOPI);
+      at(i, j, IC) = 0.0;
+    }
+  const double TwoPi = 4.0 * std::asin(1.0);
+  const double p0e = gamma0 / (2.0 * TwoPi);
+  for (int i = 0; i < isz - 1; i++)
+    for (int j = 0; j < jsz - 1; j++) {
+      const double bx = (at(i, j, IA) + at(i + 1, j, IA)) / 2;
+      const double by = (at(i, j, IB) + at(i, j + 1, IB)) / 2;
+      at(i, j, IP) = p0e / (gamma0 - 1.0) +
+                     0.5 * (at(i, j, IU) * at(i, j, IU) / at(i, j, ID) + at(i, j, IV) * at(i, j, IV) / at(i, j, ID)) +
+                     0.5 * (bx * bx + by * by);
+    }
+  const Shape s = shape_of(p);
+  U.assign((size_t)(s.cells * 8), 0.0);
+  for (int j = 0; j < s.ny; j++)
+    for (int i = 0; i < s.nx; i++)
+      for (int v = 0; v < 8; v++)
+        U[s.at(v, i, j, 0)] = at(i + gw, j + gw, v);
+}
+
+bool init_problem_mhd(const std::string& problem, const HydroParams& params, const ConfigMap& configMap,
+                      std::vector<double>& U)
+{
+  if (problem == "orszag_tang") {
+    init_orszag_tang(params, configMap, U);
+    return true;
+  }
+  return false;
 }
 
 bool init_problem(const std::string& problem, const HydroParams& params, const ConfigMap& configMap,

@@ -8,6 +8,7 @@
 //            InitImplodeFunctor2D/3D (src/muscl/HydroInitFunctors2D.h:33-122, 3D.h:68-168)
 //   blast:   BlastParams (src/shared/problems/BlastParams.h:28-53),
 //            InitBlastFunctor2D/3D (src/muscl/HydroInitFunctors2D.h:127-209, 3D.h:173-265)
+//   orszag_tang (MHD 2D): InitOrszagTangFunctor2D (src/muscl/MHDInitFunctors2D.h:345-479)
 #pragma once
 
 #include <string>
@@ -24,5 +25,10 @@ bool init_problem(const std::string& problem, const HydroParams& params, const C
 
 void init_implode(const HydroParams& params, const ConfigMap& configMap, std::vector<double>& U);
 void init_blast(const HydroParams& params, const ConfigMap& configMap, std::vector<double>& U);
+
+// MHD problems (SolverMHDMuscl<dim>::init, src/muscl/SolverMHDMuscl.h:645-705)
+bool init_problem_mhd(const std::string& problem, const HydroParams& params, const ConfigMap& configMap,
+                      std::vector<double>& U);
+void init_orszag_tang(const HydroParams& params, const ConfigMap& configMap, std::vector<double>& U);
 
 } // namespace euler_kokkos

@@ -5,6 +5,7 @@
 #include <cstdlib>
 
 #include "SolverHydroMuscl.h"
+#include "SolverMHDMuscl.h"
 
 namespace euler_kokkos {
 
@@ -12,6 +13,7 @@ SolverFactory::SolverFactory()
 {
   registerSolver("Hydro_Muscl_2D", &muscl::SolverHydroMuscl<2>::create);
   registerSolver("Hydro_Muscl_3D", &muscl::SolverHydroMuscl<3>::create);
+  registerSolver("MHD_Muscl_2D", &muscl::SolverMHDMuscl<2>::create);
 }
 
 SolverFactory& SolverFactory::Instance()

@@ -1,0 +1,142 @@
+// SolverMHDMuscl.cpp -- see SolverMHDMuscl.h.
+#include "SolverMHDMuscl.h"
+
+#include <cstdio>
+#include <iostream>
+#include <stdexcept>
+
+#include "IO_VTK.h"
+#include "Problems.h"
+
+namespace euler_kokkos {
+namespace muscl {
+
+template <int dim>
+void SolverMHDMuscl<dim>::check(int rc, const char* what) const
+{
+  if (rc != EK_OK)
+    throw std::runtime_error(std::string(what) + ": " + ek_last_error(m_ctx));
+}
+
+// src/muscl/SolverMHDMuscl.h:207-418 (constructor) and 645-705 (init)
+template <int dim>
+SolverMHDMuscl<dim>::SolverMHDMuscl(HydroParams& params, ConfigMap& configMap)
+  : SolverBase(params, configMap), isize(params.isize), jsize(params.jsize), ksize(params.ksize)
+{
+  m_nCells = dim == 2 ? (long long)params.isize * params.jsize
+                      : (long long)params.isize * params.jsize * params.ksize;
+  m_nDofsPerCell = 1;
+  const ek_params ekp = params.to_ek();
+  if (ek_create(&ekp, &m_ctx) != EK_OK)
+    throw std::runtime_error(std::string("ek_create: ") + ek_last_error(nullptr));
+  std::vector<double> U;
+  if (!init_problem_mhd(m_problem_name, params, configMap, U)) {
+    std::cout << "Problem : " << m_problem_name << " is not recognized / implemented." << std::endl;
+    std::cout << "Use default - Orszag-Tang vortex" << std::endl;
+    m_problem_name = "orszag_tang";
+    init_orszag_tang(params, configMap, U);
+  }
+  check(ek_upload(m_ctx, U.data(), EK_LAYOUT_SOA), "ek_upload");
+  compute_dt();
+
+  std::cout << "##########################" << "\n";
+  std::cout << "Solver is " << m_solver_name << "\n";
+  std::cout << "Problem (init condition) is " << m_problem_name << "\n";
+  std::cout << "##########################" << "\n";
+  params.print();
+  std::cout << "##########################" << "\n";
+  std::cout << "Memory requested : " << (ek_device_bytes(m_ctx) / 1e6) << " MBytes\n";
+  std::cout << "##########################" << "\n";
+}
+
+template <int dim>
+SolverMHDMuscl<dim>::~SolverMHDMuscl()
+{
+  ek_destroy(m_ctx);
+}
+
+// compute_dt_local: convertToPrimitives + ComputeDtFunctor2D_MHD (src/muscl/SolverMHDMuscl.h:720-737)
+template <int dim>
+double SolverMHDMuscl<dim>::compute_dt_local()
+{
+  double invDt = 0.0;
+  check(ek_inv_dt(m_ctx, &invDt), "ek_inv_dt");
+  return params.settings.cfl / invDt;
+}
+
+// src/muscl/SolverMHDMuscl.h:741-780
+template <int dim>
+void SolverMHDMuscl<dim>::next_iteration_impl()
+{
+  if (m_iteration % m_nlog == 0)
+    printf("time step=%7d (dt=% 10.8f t=% 10.8f)\n", m_iteration, m_dt, m_t);
+  if (params.enableOutput && should_save_solution()) {
+    std::cout << "Output results at time t=" << m_t << " step " << m_iteration << " dt=" << m_dt << std::endl;
+    save_solution();
+  }
+  godunov_unsplit();
+}
+
+// SolverMHDMuscl<2>::godunov_unsplit_impl (src/muscl/SolverMHDMuscl.cpp:369-454): the new dt
+// comes from the primitives of the state being advanced, then one fused launch does the rest
+template <int dim>
+void SolverMHDMuscl<dim>::godunov_unsplit()
+{
+  timers[TIMER_DT]->start();
+  compute_dt();
+  timers[TIMER_DT]->stop();
+  timers[TIMER_NUM_SCHEME]->start();
+  check(ek_step(m_ctx, m_dt), "ek_step");
+  check(ek_sync(m_ctx), "ek_sync");
+  timers[TIMER_NUM_SCHEME]->stop();
+}
+
+template <int dim>
+void SolverMHDMuscl<dim>::run()
+{
+  if (params.enableOutput) {
+    SolverBase::run();
+    return;
+  }
+  while (!finished()) {
+    if (m_iteration % m_nlog == 0)
+      printf("time step=%7d (dt=% 10.8f t=% 10.8f)\n", m_iteration, m_dt, m_t);
+    int n = m_nlog - m_iteration % m_nlog;
+    if (m_iteration + n > params.nStepmax)
+      n = params.nStepmax - m_iteration;
+    timers[TIMER_NUM_SCHEME]->start();
+    double t = m_t, dt = m_dt;
+    int it = m_iteration;
+    check(ek_run(m_ctx, n, params.nStepmax, m_tEnd, &t, &it, &dt), "ek_run");
+    timers[TIMER_NUM_SCHEME]->stop();
+    if (it == m_iteration)
+      break;
+    m_t = t;
+    m_dt = dt;
+    m_iteration = it;
+  }
+}
+
+template <int dim>
+std::vector<double> SolverMHDMuscl<dim>::state() const
+{
+  const long long n = (long long)params.nx * params.ny * (dim == 3 ? params.nz : 1) * params.nbvar;
+  std::vector<double> U((size_t)n);
+  check(ek_download(m_ctx, U.data(), EK_LAYOUT_SOA), "ek_download");
+  return U;
+}
+
+// save_solution_impl (src/muscl/SolverMHDMuscl.h:892-903): VTK output of the 8 variables
+template <int dim>
+void SolverMHDMuscl<dim>::save_solution_impl()
+{
+  timers[TIMER_IO]->start();
+  const std::vector<double> U = state();
+  save_VTK(U.data(), params, configMap, params.nbvar, m_variables_names, m_times_saved);
+  timers[TIMER_IO]->stop();
+}
+
+template class SolverMHDMuscl<2>;
+
+} // namespace muscl
+} // namespace euler_kokkos

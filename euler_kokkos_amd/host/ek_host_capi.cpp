@@ -8,6 +8,7 @@
 #include "HydroParams.h"
 #include "SolverFactory.h"
 #include "SolverHydroMuscl.h"
+#include "SolverMHDMuscl.h"
 #include "ek_host.h"
 
 using namespace euler_kokkos;
@@ -84,10 +85,12 @@ extern "C" int ek_ini_run(const char* path, double* state, long long n, double* 
     SolverBase* s = SolverFactory::Instance().create(name, p, c);
     s->run();
     std::vector<double> U;
-    if (p.dimType == TWO_D)
-      U = static_cast<muscl::SolverHydroMuscl<2>*>(s)->state();
-    else
-      U = static_cast<muscl::SolverHydroMuscl<3>*>(s)->state();
+    if (auto* h2 = dynamic_cast<muscl::SolverHydroMuscl<2>*>(s))
+      U = h2->state();
+    else if (auto* h3 = dynamic_cast<muscl::SolverHydroMuscl<3>*>(s))
+      U = h3->state();
+    else if (auto* m2 = dynamic_cast<muscl::SolverMHDMuscl<2>*>(s))
+      U = m2->state();
     if ((long long)U.size() != n) {
       delete s;
       g_err = "state buffer size mismatch";

@@ -95,3 +95,41 @@ def test_executable_writes_reference_vti(tmp_path, oracle):
         off += 8 + nbytes
     rho = arrays[0].reshape(64, 96)
     assert rho.min() > 0 and np.isfinite(rho).all()
+
+
+INI_MHD = """[run]
+solver_name=MHD_Muscl_2D
+tEnd=1e30
+nStepmax=25
+nOutput=0
+nlog=10
+[mesh]
+nx=60
+ny=44
+boundary_type_xmin=3
+boundary_type_xmax=3
+boundary_type_ymin=3
+boundary_type_ymax=3
+[hydro]
+gamma0=1.666
+cfl=0.8
+slope_type=2
+problem=orszag_tang
+riemann=hlld
+smallr=1e-7
+smallc=1e-7
+"""
+
+
+def test_ini_orszag_tang_mhd2d_matches_oracle(tmp_path, oracle):
+    ini = tmp_path / "ot.ini"
+    ini.write_text(INI_MHD)
+    ekp = make_params("MHD_Muscl_2D", 60, 44, gamma0="1.666", cfl="0.8", slope_type="2",
+                      riemann="hlld", smallr="1e-7", smallc="1e-7", bc=(3,) * 6)
+    op = oracle.from_ek(ekp)
+    U = oracle.orszag_tang(op)
+    t_o, it_o, _ = oracle.mhd_run(op, U, 25, 1e30)
+    G, t, it = ini_run(ini, 8 * 60 * 44)
+    assert it == it_o == 25 and t == t_o
+    want = U[3:-3, 3:-3, :].transpose(2, 1, 0).reshape(-1)
+    np.testing.assert_array_equal(G, want)
