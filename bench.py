@@ -131,7 +131,8 @@ def mhd_c3(steps, with_cpu, cpu_seconds):
                      "ms_per_step": wall / steps * 1e3, "avg_launch_ms": ev_ms / max(launches, 1),
                      "achieved_GBps": n * n * 2 * 8 * (8 if prec == native.EK_FP64 else 4)
                      / (ev_ms / max(launches, 1) * 1e-3) / 1e9,
-                     "max_abs_divB": float(divb), "divB_definition":
+                     "max_abs_divB": float(divb), "max_abs_divB_times_dx": float(divb * ekp.dx),
+                     "divB_definition":
                      "max over cells |(Bx(i+1,j)-Bx(i,j))/dx + (By(i,j+1)-By(i,j))/dy| on face fields",
                      "mass_drift_rel": float(abs(b1[0] - b0[0]) / abs(b0[0])),
                      "energy_drift_rel": float(abs(b1[1] - b0[1]) / abs(b0[1]))}
