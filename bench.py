@@ -158,6 +158,69 @@ def mhd_c3(steps, with_cpu, cpu_seconds):
     return res
 
 
+def batch_c4(n_runs, steps, seed):
+    """Batched 2D hydro: n_runs independent random blasts (edges 64..512) advanced together
+    by one persistent, graph-replayed pool launch per step, FP32 state with FP64 dt and
+    budgets, tiles chosen by the pool's autotuner; beside it the same batch in FP64 and
+    the plain FP64 path one job at a time (each run its own context and ek_run)."""
+    from euler_kokkos_amd import native, pool
+    runs = {prec: pool.random_blast_batch(n_runs, seed, prec) for prec in (native.EK_FP32, native.EK_FP64)}
+    cells = sum(p.nx * p.ny for p, _, _ in runs[native.EK_FP64])
+    res = {"workload": f"{n_runs} independent 2D hydro blasts, edges uniform in {{64,96,...,512}}, "
+                       f"seed {seed}, HLLC + MC, gamma 1.4, CFL 0.5, reflecting walls, {steps} steps each",
+           "runs": n_runs, "interior_cells_per_step": cells, "steps": steps, "seed": seed}
+    finals = {}
+    for name, prec in (("fp32_batched", native.EK_FP32), ("fp64_batched", native.EK_FP64)):
+        rl = runs[prec]
+        P = pool.Pool(rl[0][0], sum(2 * p.nx * p.ny * 4 * (8 if prec == native.EK_FP64 else 4) + 4096
+                                    for p, _, _ in rl) + (64 << 20), n_runs)
+        try:
+            ids = [P.admit(p, U, steps) for p, U, _ in rl]
+            choice, tune_ms = P.autotune(3)
+            device_sync()
+            t0 = time.perf_counter()
+            P.step(steps)
+            device_sync()
+            wall = time.perf_counter() - t0
+            ms, launches = P.last_step_time()
+            assert all(P.status(j)[1] == steps for j in ids)
+            finals[name] = [P.download(j)[0] for j in ids]
+        finally:
+            P.close()
+        res[name] = {"runs_per_s": n_runs / wall, "cell_updates_per_s": cells * steps / wall,
+                     "wall_s": wall, "ms_per_pool_step": ms / max(launches, 1),
+                     "tts_p50_s": wall, "tts_p99_s": wall,
+                     "tile_choice": choice, "autotune_ms_per_step": tune_ms}
+    # plain FP64, one job at a time (queue order = generation order)
+    done, per = [], []
+    tq = 0.0
+    for p, U, _ in runs[native.EK_FP64]:
+        ctx = native.Context(p)
+        try:
+            ctx.upload(U, native.EK_LAYOUT_SOA)
+            device_sync()
+            t0 = time.perf_counter()
+            ctx.run(steps, steps, 1e30)
+            device_sync()
+            el = time.perf_counter() - t0
+        finally:
+            ctx.close()
+        tq += el
+        per.append(el)
+        done.append(tq)
+    res["fp64_one_at_a_time"] = {"runs_per_s": n_runs / tq, "cell_updates_per_s": cells * steps / tq,
+                                 "wall_s": tq, "tts_p50_s": float(np.percentile(done, 50)),
+                                 "tts_p99_s": float(np.percentile(done, 99)),
+                                 "run_time_p50_s": float(np.percentile(per, 50)),
+                                 "run_time_p99_s": float(np.percentile(per, 99))}
+    l1 = [float(np.abs(a - b).mean() / np.abs(b).mean()) for a, b in zip(finals["fp32_batched"], finals["fp64_batched"])]
+    res["fp32_l1_rho_rel_max"] = max(l1)
+    res["fp32_l1_rho_rel_mean"] = float(np.mean(l1))
+    res["fp32_batched_vs_fp64_one_at_a_time"] = (res["fp32_batched"]["cell_updates_per_s"]
+                                                 / res["fp64_one_at_a_time"]["cell_updates_per_s"])
+    return res
+
+
 def copy_calibration(gib=1.0):
     """A device-to-device copy of known size, for scaling rocprofv3 FETCH/WRITE counters."""
     hip = C.CDLL("libamdhip64.so")
@@ -186,6 +249,10 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-mhd", action="store_true", help="skip the 2D MHD (Orszag-Tang) line")
     ap.add_argument("--mhd-steps", type=int, default=500)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D hydro line")
+    ap.add_argument("--batch-runs", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--batch-seed", type=int, default=2024)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build has no multi-GPU "
@@ -248,6 +315,8 @@ def main() -> int:
         out["cpu_baseline"] = None
     if not a.no_mhd:
         out["mhd2d"] = mhd_c3(a.mhd_steps, not a.no_cpu, a.cpu_seconds / 2)
+    if not a.no_batch:
+        out["batch2d"] = batch_c4(a.batch_runs, a.batch_steps, a.batch_seed)
     print(json.dumps(out))
     return 0
 

@@ -46,6 +46,9 @@ UNITS = [
     ("ek_capi.hip", ["-ffp-contract=off"]),
     ("ek_mhd_fp64.hip", ["-ffp-contract=off"]),
     ("ek_mhd_fp32.hip", FP32_FLAGS),
+    ("ek_pool_fp64.hip", ["-ffp-contract=off"]),
+    ("ek_pool_fp32.hip", FP32_FLAGS),
+    ("ek_pool.hip", ["-ffp-contract=off"]),
 ]
 HOST_UNITS = ["ConfigMap.cpp", "HydroParams.cpp", "SolverBase.cpp", "SolverHydroMuscl.cpp", "SolverMHDMuscl.cpp",
               "SolverFactory.cpp", "Problems.cpp", "IO_VTK.cpp", "ek_host_capi.cpp"]

@@ -2,6 +2,7 @@
 // device-resident time loop (captured into a HIP graph) and the budget reductions.
 #include "ek_hip.h"
 #include "ek_internal.h"
+#include "ek_geom_host.h"
 
 #include <cmath>
 #include <cstdio>
@@ -64,46 +65,6 @@ static int hip_fail(ek_ctx* c, hipError_t e, const char* what)
       return hip_fail((c), e_, #call);       \
   } while (0)
 
-static bool is_pow2(double x)
-{
-  if (!(x > 0.0) || !std::isfinite(x))
-    return false;
-  int e;
-  const double m = std::frexp(x, &e);
-  return m == 0.5;
-}
-
-template <class T>
-static void fill_geom(const ek_params& p, ek::Geom<T>& g, ek::Gas<T>& gas)
-{
-  g.nx = p.nx;
-  g.ny = p.ny;
-  g.nz = p.dim == 3 ? p.nz : 1;
-  for (int i = 0; i < 6; i++)
-    g.bc[i] = p.bc[i];
-  g.plane = (long long)p.nx * p.ny;
-  g.vstride = g.plane * g.nz;
-  g.dx = p.dx;
-  g.dy = p.dy;
-  g.dz = p.dz;
-  g.tdx = (T)p.dx;
-  g.tdy = (T)p.dy;
-  g.tdz = (T)p.dz;
-  g.pow2 = is_pow2(p.dx) && is_pow2(p.dy) && (p.dim == 2 || is_pow2(p.dz));
-  g.idx = (T)(1.0 / p.dx);
-  g.idy = (T)(1.0 / p.dy);
-  g.idz = (T)(1.0 / p.dz);
-  gas.gamma0 = (T)p.gamma0;
-  gas.gamma6 = (T)p.gamma6;
-  gas.gm1 = (T)(p.gamma0 - 1.0);
-  gas.smallr = (T)p.smallr;
-  gas.smallc = (T)p.smallc;
-  gas.smallp = (T)p.smallp;
-  gas.smallpp = (T)p.smallpp;
-  gas.smallc2 = (T)(p.smallc * p.smallc);
-  gas.entho = (T)(1.0 / (p.gamma0 - 1.0));
-}
-
 static int validate_common(const ek_params* p, std::string& why)
 {
   if (p->precision != EK_FP64 && p->precision != EK_FP32)
@@ -118,7 +79,7 @@ static int validate_common(const ek_params* p, std::string& why)
   return 0;
 }
 
-static int validate(const ek_params* p, std::string& why)
+int ek::validate_params(const ek_params* p, std::string& why)
 {
   if (p->dim != 2 && p->dim != 3)
     return why = "dim must be 2 or 3", 1;
@@ -158,14 +119,14 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
     return fail(nullptr, EK_EINVAL, "null argument");
   *out = nullptr;
   std::string why;
-  if (validate(params, why))
+  if (ek::validate_params(params, why))
     return fail(nullptr, EK_EINVAL, why);
   ek_ctx* c = new ek_ctx();
   c->p = *params;
   c->st = (int)params->slope_type;
   c->elem = params->precision == EK_FP64 ? 8 : 4;
-  fill_geom<double>(c->p, c->g64, c->gas64);
-  fill_geom<float>(c->p, c->g32, c->gas32);
+  ek::fill_geom<double>(c->p, c->g64, c->gas64);
+  ek::fill_geom<float>(c->p, c->g32, c->gas32);
   c->ncell = c->g64.vstride;
   c->cz = params->dim == 3 ? (params->nz < 32 ? params->nz : 32) : 1;
   hipError_t e;

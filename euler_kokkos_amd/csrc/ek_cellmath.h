@@ -56,7 +56,20 @@ __host__ __device__ inline void prims(const Gas<T>& g, const T* u, T* q)
   } else {
     eken = kHalf<T>() * (ux * ux + uy * uy);
   }
-  const T e = u[IP] / d - eken;
+  T e = u[IP] / d - eken;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 state: when the internal energy is a small remainder of E/rho (more than 7 of
+    // the 24 mantissa bits cancelled), recover it in FP64 from the same conserved values.
+    if (e < T(0.0078125) * (u[IP] / d)) {
+      const double dd = d, ix = (double)u[IU] / dd, iy = (double)u[IV] / dd;
+      double ek = ix * ix + iy * iy;
+      if constexpr (DIM == 3) {
+        const double iz = (double)u[IW] / dd;
+        ek += iz * iz;
+      }
+      e = (T)((double)u[IP] / dd - 0.5 * ek);
+    }
+  }
   q[ID] = d;
   q[IP] = emax(g.gm1 * d * e, d * g.smallp);
   q[IU] = ux;

@@ -202,32 +202,52 @@ __device__ __forceinline__ void step_end_bookkeeping(const StepCtl& c, const Tim
   c.ts[nn].invdt_bits = 0ull;
 }
 
+// CFL term of one cell of the new state. The FP64 state uses the reference's expression
+// as is; an FP32 state is widened so that the dt reduction always runs in FP64.
+template <class T, int DIM>
+__device__ __forceinline__ double cfl_term(const Gas<T>& gas, const T* u, const Geom<T>& geo)
+{
+  if constexpr (sizeof(T) == 8) {
+    return (double)cell_inv_dt<T, DIM>(gas, u, geo.tdx, geo.tdy, geo.tdz, geo.idx, geo.idy, geo.idz,
+                                       geo.pow2 != 0);
+  } else {
+    constexpr int NV = DIM == 3 ? 5 : 4;
+    Gas<double> g;
+    g.gamma0 = gas.gamma0;
+    g.gm1 = gas.gm1;
+    g.smallr = gas.smallr;
+    g.smallp = gas.smallp;
+    double ud[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      ud[v] = (double)u[v];
+    return cell_inv_dt<double, DIM>(g, ud, geo.dx, geo.dy, geo.dz, 1.0 / geo.dx, 1.0 / geo.dy,
+                                    1.0 / geo.dz, false);
+  }
+}
+
 // ------------------------------------------------------------------------------------
 // 2D: one block per TX x TY tile. Work items per tile: primitives on (TX+4)x(TY+4)
 // positions, slopes+trace on the tile and its 1-wide halo cross, (TX+1)*TY x-faces and
 // TX*(TY+1) y-faces, then the update of the TX*TY cells. Every face flux is solved once
 // per tile; slopes are computed once per position (the reference recomputes them up to 3x).
 // ------------------------------------------------------------------------------------
+// One TX x TY tile of the 2D step, origin (i0, j0), with time step dt. Returns this
+// thread's contribution to the CFL maximum of the new state. Shared by the single-grid
+// kernel below and the batched (job pool) kernel.
 template <class T, int TX, int TY, int NT, int RS, int ST>
-__global__ __launch_bounds__(NT) void hydro2d_step(const T* __restrict__ Uin, T* __restrict__ Uout,
-                                                   Geom<T> geo, Gas<T> gas, StepCtl ctl)
+__device__ __forceinline__ double hydro2d_tile(const T* __restrict__ Uin, T* __restrict__ Uout,
+                                               const Geom<T>& geo, const Gas<T>& gas, double dt, int i0, int j0)
 {
   constexpr int NV = 4;
   constexpr int QX = TX + 4, QY = TY + 4, QN = QX * QY;
   __shared__ T sQ[NV][QN];
   __shared__ T sX[2][NV][TY][TX + 1]; // [0]: left state at face f (cell f-1), [1]: right state (cell f)
   __shared__ T sY[2][NV][TY + 1][TX];
-  __shared__ double sred[NT / 64];
 
   const int tid = threadIdx.x;
-  const bool first_block = blockIdx.x == 0 && blockIdx.y == 0;
-  double dt;
-  TimeSlot cur;
-  if (!step_begin(ctl, first_block, dt, cur))
-    return;
   const T dtdx[2] = { T(dt / geo.dx), T(dt / geo.dy) };
   const int nx = geo.nx, ny = geo.ny;
-  const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
 
   // primitives of the tile plus a 2-cell halo, ghosts resolved through the boundary map
   for (int w = tid; w < QN; w += NT) {
@@ -364,13 +384,27 @@ __global__ __launch_bounds__(NT) void hydro2d_step(const T* __restrict__ Uin, T*
         u[v] = a;
         Uout[v * geo.vstride + o] = a;
       }
-      const double s = (double)cell_inv_dt<T, 2>(gas, u, geo.tdx, geo.tdy, geo.tdz, geo.idx, geo.idy,
-                                                 geo.idz, geo.pow2 != 0);
+      const double s = cfl_term<T, 2>(gas, u, geo);
       local = (s > local || !(s == s)) ? s : local;
     }
   }
+  return local;
+}
+
+template <class T, int TX, int TY, int NT, int RS, int ST>
+__global__ __launch_bounds__(NT) void hydro2d_step(const T* __restrict__ Uin, T* __restrict__ Uout,
+                                                   Geom<T> geo, Gas<T> gas, StepCtl ctl)
+{
+  __shared__ double sred[NT / 64];
+  const bool first_block = blockIdx.x == 0 && blockIdx.y == 0;
+  double dt;
+  TimeSlot cur;
+  if (!step_begin(ctl, first_block, dt, cur))
+    return;
+  const double local =
+      hydro2d_tile<T, TX, TY, NT, RS, ST>(Uin, Uout, geo, gas, dt, blockIdx.x * TX, blockIdx.y * TY);
   block_max_to<NT>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
-  if (first_block && tid == 0)
+  if (first_block && threadIdx.x == 0)
     step_end_bookkeeping(ctl, cur, dt);
 }
 
@@ -389,8 +423,7 @@ __global__ __launch_bounds__(NT) void inv_dt_kernel(const T* __restrict__ U, Geo
 #pragma unroll
     for (int v = 0; v < NV; v++)
       u[v] = U[v * geo.vstride + c];
-    const double s = (double)cell_inv_dt<T, DIM>(gas, u, geo.tdx, geo.tdy, geo.tdz, geo.idx, geo.idy,
-                                                 geo.idz, geo.pow2 != 0);
+    const double s = cfl_term<T, DIM>(gas, u, geo);
     local = (s > local || !(s == s)) ? s : local;
   }
   block_max_to<NT>(dst, local, sred);

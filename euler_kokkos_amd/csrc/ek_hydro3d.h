@@ -145,10 +145,13 @@ __device__ __forceinline__ constexpr int rot_z(int v)
   return v == IU ? IW : (v == IW ? IU : v);
 }
 
+// One TX x TY column tile over planes [k0, k0+CZ) of the 3D step with time step dt.
+// Returns this thread's contribution to the CFL maximum of the new state. Shared by the
+// single-grid kernel below and the batched (job pool) kernel.
 template <class T, int TX, int TY, int NT, int RS, int ST>
-__global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T* __restrict__ Uout,
-                                                   Geom<T> geo, Gas<T> gas, StepCtl ctl, int CZ,
-                                                   int ntx, int nty, int nchunk)
+__device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* __restrict__ Uout,
+                                                 const Geom<T>& geo, const Gas<T>& gas, double dt, int i0,
+                                                 int j0, int k0, int CZ)
 {
   using C = Col3<TX, TY, NT>;
   constexpr int NV = 5;
@@ -161,27 +164,11 @@ __global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T*
   // Faces: x (fy*(TX+1)+fx), then y (NFX + fy*TX+fx), then z (NFX+NFY + tile cell).
   // After the solve, [0][.][f] holds the face flux times dt/dx, still rotated.
   __shared__ T sF[2][NV][NF];
-  __shared__ double sred[NT / 64];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  // XCD-aware tile order: blocks are dispatched round-robin over the 8 XCDs, so give each
-  // XCD a contiguous range of tiles (neighbours share halo lines in that XCD's L2).
-  const int nblk = ntx * nty * nchunk;
-  const int b = blockIdx.x;
-  const int per = nblk / 8;
-  const int lb = (per * 8 == nblk) ? (b % 8) * per + b / 8 : b;
-  const int bx = lb % ntx, by = (lb / ntx) % nty, bz = lb / (ntx * nty);
-  const bool first_block = b == 0;
-
-  double dt;
-  TimeSlot cur;
-  if (!step_begin(ctl, first_block, dt, cur))
-    return;
   const T dtdx[3] = { T(dt / geo.dx), T(dt / geo.dy), T(dt / geo.dz) };
   const int nx = geo.nx, ny = geo.ny, nz = geo.nz;
-  const int i0 = bx * TX, j0 = by * TY;
-  const int k0 = bz * CZ;
   const int k1 = min(k0 + CZ, nz);
 
   int qx, qy;
@@ -355,8 +342,7 @@ __global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T*
 #pragma unroll
           for (int v = 0; v < NV; v++)
             Uout[v * geo.vstride + o] = u[v];
-          const double s = (double)cell_inv_dt<T, 3>(gas, u, geo.tdx, geo.tdy, geo.tdz, geo.idx,
-                                                     geo.idy, geo.idz, geo.pow2 != 0);
+          const double s = cfl_term<T, 3>(gas, u, geo);
           local = (s > local || !(s == s)) ? s : local;
         }
       }
@@ -380,8 +366,31 @@ __global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T*
       qc[v] = qn[v];
     }
   }
+  return local;
+}
+
+template <class T, int TX, int TY, int NT, int RS, int ST>
+__global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T* __restrict__ Uout,
+                                                   Geom<T> geo, Gas<T> gas, StepCtl ctl, int CZ,
+                                                   int ntx, int nty, int nchunk)
+{
+  __shared__ double sred[NT / 64];
+  // XCD-aware tile order: blocks are dispatched round-robin over the 8 XCDs, so give each
+  // XCD a contiguous range of tiles (neighbours share halo lines in that XCD's L2).
+  const int nblk = ntx * nty * nchunk;
+  const int b = blockIdx.x;
+  const int per = nblk / 8;
+  const int lb = (per * 8 == nblk) ? (b % 8) * per + b / 8 : b;
+  const int bx = lb % ntx, by = (lb / ntx) % nty, bz = lb / (ntx * nty);
+  const bool first_block = b == 0;
+  double dt;
+  TimeSlot cur;
+  if (!step_begin(ctl, first_block, dt, cur))
+    return;
+  const double local =
+      hydro3d_column<T, TX, TY, NT, RS, ST>(Uin, Uout, geo, gas, dt, bx * TX, by * TY, bz * CZ, CZ);
   block_max_to<NT>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
-  if (first_block && tid == 0)
+  if (first_block && threadIdx.x == 0)
     step_end_bookkeeping(ctl, cur, dt);
 }
 

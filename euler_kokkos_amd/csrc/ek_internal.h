@@ -4,8 +4,16 @@
 #pragma once
 
 #include "ek_hydro.h"
+#include "ek_pool_kernels.h"
+
+#include <string>
+
+#include "ek_hip.h"
 
 namespace ek {
+// Checks an ek_params against what this library provides (ek_capi.hip); 0 when valid.
+int validate_params(const ek_params* p, std::string& why);
+
 namespace fp64 {
 hipError_t step(int dim, int rs, int st, const double* Uin, double* Uout, const Geom<double>& geo,
                 const Gas<double>& gas, const StepCtl& ctl, int cz, hipStream_t s);
@@ -22,6 +30,10 @@ hipError_t mhd_step(int rs, int st, const double* Uin, double* Uout, const Geom<
                     const StepCtl& ctl, hipStream_t s);
 hipError_t mhd_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
                       hipStream_t s);
+// job pool (ek_pool_fp64.hip)
+hipError_t pool_step(int dim, int variant, int st, const double* Ua, double* Ub, const PoolJob<double>* jobs,
+                     const PoolWork* work, int nwork, TimeSlot* ts, int slot, int grid, hipStream_t s);
+int pool_occupancy(int dim, int variant, int st);
 } // namespace fp64
 namespace fp32 {
 hipError_t step(int dim, int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo,
@@ -39,5 +51,9 @@ hipError_t mhd_step(int rs, int st, const float* Uin, float* Uout, const Geom<fl
                     const StepCtl& ctl, hipStream_t s);
 hipError_t mhd_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
                       hipStream_t s);
+// job pool (ek_pool_fp32.hip)
+hipError_t pool_step(int dim, int variant, int st, const float* Ua, float* Ub, const PoolJob<float>* jobs,
+                     const PoolWork* work, int nwork, TimeSlot* ts, int slot, int grid, hipStream_t s);
+int pool_occupancy(int dim, int variant, int st);
 } // namespace fp32
 } // namespace ek

@@ -43,7 +43,18 @@ __host__ __device__ inline void constoprim(const Gas<T>& g, const T* u, T mx, T 
   q[IC] = T(0.5) * (u[IC] + mz);
   const T eken = T(0.5) * (q[IU] * q[IU] + q[IV] * q[IV] + q[IW] * q[IW]);
   const T emag = T(0.5) * (q[IA] * q[IA] + q[IB] * q[IB] + q[IC] * q[IC]);
-  const T eint = (u[IP] - emag) / q[ID] - eken;
+  T eint = (u[IP] - emag) / q[ID] - eken;
+  if constexpr (sizeof(T) == 4) {
+    // FP32 state: p = (gamma-1)(E - rho v^2/2 - B^2/2) recovered in FP64 where more than 7
+    // of the 24 mantissa bits cancel
+    if (eint < T(0.0078125) * (u[IP] / q[ID])) {
+      const double d = q[ID], ux = (double)u[IU] / d, uy = (double)u[IV] / d, uz = (double)u[IW] / d;
+      const double bx = 0.5 * ((double)u[IA] + (double)mx), by = 0.5 * ((double)u[IB] + (double)my),
+                   bz = 0.5 * ((double)u[IC] + (double)mz);
+      const double em = 0.5 * (bx * bx + by * by + bz * bz);
+      eint = (T)(((double)u[IP] - em) / d - 0.5 * (ux * ux + uy * uy + uz * uz));
+    }
+  }
   q[IP] = emax(g.gm1 * q[ID] * eint, q[ID] * g.smallp);
 }
 
@@ -75,6 +86,31 @@ __host__ __device__ inline T cell_inv_dt(const Gas<T>& g, const T* q, T dx, T dy
   if (pow2)
     return vx * idx + vy * idy;
   return vx / dx + vy / dy;
+}
+
+// CFL term of one cell: primitives (with the right/top face fields) then fast speeds. The
+// FP64 state uses the reference's expressions; an FP32 state is widened so that the dt
+// reduction always runs in FP64.
+template <class T>
+__device__ __forceinline__ double mhd_cfl_term(const Gas<T>& gas, const T* u, T mx, T my, const Geom<T>& geo)
+{
+  if constexpr (sizeof(T) == 8) {
+    T q[NV];
+    constoprim<T>(gas, u, mx, my, T(0), q);
+    return cell_inv_dt<T>(gas, q, geo.tdx, geo.tdy, geo.idx, geo.idy, geo.pow2 != 0);
+  } else {
+    Gas<double> g;
+    g.gamma0 = gas.gamma0;
+    g.gm1 = gas.gm1;
+    g.smallr = gas.smallr;
+    g.smallp = gas.smallp;
+    double ud[NV], q[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      ud[v] = (double)u[v];
+    constoprim<double>(g, ud, (double)mx, (double)my, 0.0, q);
+    return cell_inv_dt<double>(g, q, geo.dx, geo.dy, 1.0 / geo.dx, 1.0 / geo.dy, false);
+  }
 }
 
 // find_mhd_flux (mhd_utils.h:173-231)
@@ -824,9 +860,7 @@ __global__ __launch_bounds__(NT) void mhd2d_step(const T* __restrict__ Uin, T* _
       } else {
         bt = u[IB];
       }
-      T q[NV];
-      constoprim<T>(gas, u, ar, bt, T(0), q);
-      const double s = (double)cell_inv_dt<T>(gas, q, geo.tdx, geo.tdy, geo.idx, geo.idy, geo.pow2 != 0);
+      const double s = mhd_cfl_term<T>(gas, u, ar, bt, geo);
       local = (s > local || !(s == s)) ? s : local;
     }
   }
@@ -862,9 +896,7 @@ __global__ __launch_bounds__(NT) void mhd2d_inv_dt_kernel(const T* __restrict__ 
     T my = U[IB * vs + (long long)sj * nx + i];
     if (fy)
       my = my * T(-1);
-    T q[NV];
-    constoprim<T>(gas, u, mx, my, T(0), q);
-    const double s = (double)cell_inv_dt<T>(gas, q, geo.tdx, geo.tdy, geo.idx, geo.idy, geo.pow2 != 0);
+    const double s = mhd_cfl_term<T>(gas, u, mx, my, geo);
     local = (s > local || !(s == s)) ? s : local;
   }
   block_max_to<NT>(dst, local, sred);

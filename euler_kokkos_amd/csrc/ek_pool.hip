@@ -1,0 +1,732 @@
+// ek_pool.hip -- the job pool of include/ek_pool.h: arena allocation, admission and
+// retirement, the work list, graph-captured pool steps and the tile autotuner.
+#include "ek_pool.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ek_geom_host.h"
+#include "ek_internal.h"
+#include "ek_pool_launch.h"
+
+namespace {
+
+struct HostJob {
+  bool used = false;
+  ek_params p{};
+  long long off = 0, nelem = 0, alloc = 0;
+  long long adm_step = 0;
+  int nStepmax = 0;
+  double tEnd = 0.0;
+  double t = 0.0;
+  int it = 0;
+  int finished = 0;
+  long long ncell = 0;
+};
+
+constexpr long long kAlign = 64; // elements
+constexpr int kGraphSteps = 6;   // lcm of the 2 arena buffers and the 3 time slots
+constexpr int kCz[4] = { 8, 16, 32, 64 };
+
+long long round_up(long long n)
+{
+  return (n + kAlign - 1) / kAlign * kAlign;
+}
+
+} // namespace
+
+struct ek_pool {
+  ek_params kind{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  bool fp64 = true;
+  size_t elem = 8;
+  long long cap = 0; // elements per arena buffer
+  void* A[2] = { nullptr, nullptr };
+  ek::TimeSlot* ts = nullptr;
+  int max_jobs = 0;
+  void* djobs = nullptr;
+  ek::PoolWork* dwork = nullptr;
+  int work_cap = 0;
+  int nwork = 0;
+  std::vector<HostJob> jobs;
+  std::map<long long, long long> freel; // offset -> length (elements)
+  long long nstep = 0;
+  int variant = 0;
+  int cus = 1;
+  int grid = 1;
+  hipGraphExec_t graph = nullptr;
+  bool graph_ok = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int last_launches = 0;
+  double* stage = nullptr;
+  long long stage_elems = 0;
+  double* partial = nullptr;
+};
+
+static int pfail(ek_pool* p, int code, const std::string& m)
+{
+  if (p)
+    p->err = m;
+  return code;
+}
+
+#define PHIP(p, call)                                                  \
+  do {                                                                 \
+    hipError_t e_ = (call);                                            \
+    if (e_ != hipSuccess)                                              \
+      return pfail((p), EK_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static std::string g_pool_create_err;
+
+static int kind_dim(const ek_pool* p)
+{
+  return p->kind.dim;
+}
+
+static int cz_of(const ek_pool* p, const ek_params& j)
+{
+  const int c = kCz[p->variant & 3];
+  return c < j.nz ? c : j.nz;
+}
+
+// PoolJob record of job `id` for the current tile choice
+template <class T>
+static ek::PoolJob<T> make_record(const ek_pool* p, int id)
+{
+  const HostJob& h = p->jobs[id];
+  ek::PoolJob<T> r{};
+  r.off = h.off;
+  r.slot0 = 3 * id;
+  r.nStepmax = h.nStepmax;
+  r.cz = h.p.dim == 3 ? cz_of(p, h.p) : 1;
+  r.cfl = h.p.cfl;
+  r.tEnd = h.tEnd;
+  ek::fill_geom<T>(h.p, r.geo, r.gas);
+  return r;
+}
+
+static int upload_record(ek_pool* p, int id)
+{
+  if (p->fp64) {
+    const ek::PoolJob<double> r = make_record<double>(p, id);
+    PHIP(p, hipMemcpyAsync((ek::PoolJob<double>*)p->djobs + id, &r, sizeof(r), hipMemcpyHostToDevice, p->stream));
+  } else {
+    const ek::PoolJob<float> r = make_record<float>(p, id);
+    PHIP(p, hipMemcpyAsync((ek::PoolJob<float>*)p->djobs + id, &r, sizeof(r), hipMemcpyHostToDevice, p->stream));
+  }
+  PHIP(p, hipStreamSynchronize(p->stream));
+  return EK_OK;
+}
+
+static void tiles_of(const ek_pool* p, const ek_params& j, int& ntx, int& nty, int& ntz)
+{
+  if (j.dim == 2) {
+    ntx = (j.nx + ek::pool::V2[p->variant][0] - 1) / ek::pool::V2[p->variant][0];
+    nty = (j.ny + ek::pool::V2[p->variant][1] - 1) / ek::pool::V2[p->variant][1];
+    ntz = 1;
+  } else {
+    ntx = (j.nx + ek::pool::P3X - 1) / ek::pool::P3X;
+    nty = (j.ny + ek::pool::P3Y - 1) / ek::pool::P3Y;
+    const int cz = cz_of(p, j);
+    ntz = (j.nz + cz - 1) / cz;
+  }
+}
+
+static int rebuild_work(ek_pool* p)
+{
+  std::vector<ek::PoolWork> w;
+  for (int id = 0; id < (int)p->jobs.size(); id++) {
+    const HostJob& h = p->jobs[id];
+    if (!h.used || h.finished)
+      continue;
+    int ntx, nty, ntz;
+    tiles_of(p, h.p, ntx, nty, ntz);
+    for (int tz = 0; tz < ntz; tz++)
+      for (int ty = 0; ty < nty; ty++)
+        for (int tx = 0; tx < ntx; tx++)
+          w.push_back(ek::PoolWork{ id, tx, ty, tz * 2 + ((tx | ty | tz) == 0 ? 1 : 0) });
+  }
+  if ((int)w.size() > p->work_cap) {
+    hipFree(p->dwork);
+    p->dwork = nullptr;
+    p->work_cap = 0;
+    const int cap = (int)std::max<size_t>(w.size() * 2, 1024);
+    PHIP(p, hipMalloc(&p->dwork, (size_t)cap * sizeof(ek::PoolWork)));
+    p->work_cap = cap;
+  }
+  if (!w.empty())
+    PHIP(p, hipMemcpyAsync(p->dwork, w.data(), w.size() * sizeof(ek::PoolWork), hipMemcpyHostToDevice, p->stream));
+  PHIP(p, hipStreamSynchronize(p->stream));
+  p->nwork = (int)w.size();
+  p->graph_ok = false;
+  return EK_OK;
+}
+
+static void set_grid(ek_pool* p)
+{
+  const int occ = p->fp64 ? ek::fp64::pool_occupancy(kind_dim(p), p->variant, (int)p->kind.slope_type)
+                          : ek::fp32::pool_occupancy(kind_dim(p), p->variant, (int)p->kind.slope_type);
+  p->grid = p->cus * occ;
+}
+
+static hipError_t launch_one(ek_pool* p, long long n)
+{
+  const int par = (int)(n % 2), slot = (int)(n % 3);
+  const int st = (int)p->kind.slope_type;
+  if (p->fp64)
+    return ek::fp64::pool_step(kind_dim(p), p->variant, st, (const double*)p->A[par], (double*)p->A[1 - par],
+                               (const ek::PoolJob<double>*)p->djobs, p->dwork, p->nwork, p->ts, slot, p->grid,
+                               p->stream);
+  return ek::fp32::pool_step(kind_dim(p), p->variant, st, (const float*)p->A[par], (float*)p->A[1 - par],
+                             (const ek::PoolJob<float>*)p->djobs, p->dwork, p->nwork, p->ts, slot, p->grid,
+                             p->stream);
+}
+
+static int capture_graph(ek_pool* p)
+{
+  if (p->graph) {
+    hipGraphExecDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  PHIP(p, hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed));
+  for (int i = 0; i < kGraphSteps; i++) {
+    hipError_t e = launch_one(p, i);
+    if (e != hipSuccess) {
+      hipStreamEndCapture(p->stream, &g);
+      if (g)
+        hipGraphDestroy(g);
+      return pfail(p, EK_EHIP, std::string("pool graph capture: ") + hipGetErrorString(e));
+    }
+  }
+  PHIP(p, hipStreamEndCapture(p->stream, &g));
+  hipError_t e = hipGraphInstantiate(&p->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return pfail(p, EK_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  p->graph_ok = true;
+  return EK_OK;
+}
+
+static int refresh_status(ek_pool* p, bool& changed)
+{
+  changed = false;
+  std::vector<ek::TimeSlot> h((size_t)p->max_jobs * 3);
+  PHIP(p, hipMemcpyAsync(h.data(), p->ts, h.size() * sizeof(ek::TimeSlot), hipMemcpyDeviceToHost, p->stream));
+  PHIP(p, hipStreamSynchronize(p->stream));
+  const int slot = (int)(p->nstep % 3);
+  for (int id = 0; id < (int)p->jobs.size(); id++) {
+    HostJob& j = p->jobs[id];
+    if (!j.used)
+      continue;
+    const ek::TimeSlot& s = h[(size_t)id * 3 + slot];
+    j.t = s.t;
+    j.it = s.it;
+    const int fin = (s.finished || s.t >= (j.tEnd - 1e-14) || s.it >= j.nStepmax) ? 1 : 0;
+    if (fin && !j.finished)
+      changed = true;
+    j.finished = fin;
+  }
+  return EK_OK;
+}
+
+// arena ------------------------------------------------------------------------------
+
+static long long arena_alloc(ek_pool* p, long long n)
+{
+  for (auto it = p->freel.begin(); it != p->freel.end(); ++it) {
+    if (it->second >= n) {
+      const long long off = it->first, len = it->second;
+      p->freel.erase(it);
+      if (len > n)
+        p->freel[off + n] = len - n;
+      return off;
+    }
+  }
+  return -1;
+}
+
+static void arena_free(ek_pool* p, long long off, long long n)
+{
+  auto it = p->freel.emplace(off, n).first;
+  auto nx = std::next(it);
+  if (nx != p->freel.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    p->freel.erase(nx);
+  }
+  if (it != p->freel.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      p->freel.erase(it);
+    }
+  }
+}
+
+static int ensure_stage(ek_pool* p, long long n)
+{
+  if (p->stage_elems >= n)
+    return EK_OK;
+  hipFree(p->stage);
+  p->stage = nullptr;
+  p->stage_elems = 0;
+  PHIP(p, hipMalloc(&p->stage, (size_t)n * sizeof(double)));
+  p->stage_elems = n;
+  return EK_OK;
+}
+
+static int job_parity(const ek_pool* p, const HostJob& j)
+{
+  return (int)((j.adm_step + j.it) % 2);
+}
+
+// C ABI ------------------------------------------------------------------------------
+
+extern "C" int ek_pool_create(const ek_params* kind, long long arena_bytes, int max_jobs, ek_pool** out)
+{
+  if (!kind || !out || arena_bytes <= 0 || max_jobs <= 0) {
+    g_pool_create_err = "invalid argument";
+    return EK_EINVAL;
+  }
+  *out = nullptr;
+  std::string why;
+  if (ek::validate_params(kind, why)) {
+    g_pool_create_err = why;
+    return EK_EINVAL;
+  }
+  if (kind->mhd || kind->riemann != ek::RS_HLLC || !(kind->slope_type == 1.0 || kind->slope_type == 2.0)) {
+    g_pool_create_err = "the job pool runs hydro HLLC jobs with slope_type 1 or 2";
+    return EK_EINVAL;
+  }
+  ek_pool* p = new ek_pool();
+  p->kind = *kind;
+  p->fp64 = kind->precision == EK_FP64;
+  p->elem = p->fp64 ? 8 : 4;
+  p->cap = arena_bytes / 2 / (long long)p->elem / kAlign * kAlign;
+  p->max_jobs = max_jobs;
+  p->jobs.resize(max_jobs);
+  p->freel[0] = p->cap;
+  p->variant = kind->dim == 2 ? 0 : 2; // 16x16 tiles (2D), 32-plane chunks (3D) until tuned
+  int dev = 0;
+  hipError_t e;
+  const size_t recb = p->fp64 ? sizeof(ek::PoolJob<double>) : sizeof(ek::PoolJob<float>);
+  if ((e = hipGetDevice(&dev)) != hipSuccess ||
+      (e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc(&p->A[0], (size_t)p->cap * p->elem)) != hipSuccess ||
+      (e = hipMalloc(&p->A[1], (size_t)p->cap * p->elem)) != hipSuccess ||
+      (e = hipMalloc(&p->ts, (size_t)max_jobs * 3 * sizeof(ek::TimeSlot))) != hipSuccess ||
+      (e = hipMalloc(&p->djobs, (size_t)max_jobs * recb)) != hipSuccess ||
+      (e = hipMalloc(&p->partial, (size_t)1024 * 16 * sizeof(double))) != hipSuccess ||
+      (e = hipEventCreate(&p->ev0)) != hipSuccess || (e = hipEventCreate(&p->ev1)) != hipSuccess ||
+      (e = hipMemsetAsync(p->ts, 0, (size_t)max_jobs * 3 * sizeof(ek::TimeSlot), p->stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(p->stream)) != hipSuccess) {
+    g_pool_create_err = std::string("pool allocation: ") + hipGetErrorString(e);
+    ek_pool_destroy(p);
+    return e == hipErrorOutOfMemory ? EK_ENOMEM : EK_EHIP;
+  }
+  set_grid(p);
+  *out = p;
+  return EK_OK;
+}
+
+extern "C" void ek_pool_destroy(ek_pool* p)
+{
+  if (!p)
+    return;
+  if (p->stream)
+    hipStreamSynchronize(p->stream);
+  if (p->graph)
+    hipGraphExecDestroy(p->graph);
+  if (p->ev0)
+    hipEventDestroy(p->ev0);
+  if (p->ev1)
+    hipEventDestroy(p->ev1);
+  hipFree(p->A[0]);
+  hipFree(p->A[1]);
+  hipFree(p->ts);
+  hipFree(p->djobs);
+  hipFree(p->dwork);
+  hipFree(p->stage);
+  hipFree(p->partial);
+  if (p->stream)
+    hipStreamDestroy(p->stream);
+  delete p;
+}
+
+extern "C" const char* ek_pool_last_error(const ek_pool* p)
+{
+  return p ? p->err.c_str() : g_pool_create_err.c_str();
+}
+
+static long long job_cells(const ek_params& j)
+{
+  return (long long)j.nx * j.ny * (j.dim == 3 ? j.nz : 1);
+}
+
+extern "C" long long ek_pool_job_bytes(const ek_pool* p, const ek_params* j)
+{
+  if (!p || !j)
+    return -1;
+  return 2 * round_up(job_cells(*j) * j->nbvar) * (long long)p->elem;
+}
+
+extern "C" long long ek_pool_free_bytes(const ek_pool* p)
+{
+  if (!p)
+    return -1;
+  long long f = 0;
+  for (const auto& kv : p->freel)
+    f += kv.second;
+  return 2 * f * (long long)p->elem;
+}
+
+extern "C" int ek_pool_active(const ek_pool* p)
+{
+  if (!p)
+    return 0;
+  int n = 0;
+  for (const HostJob& j : p->jobs)
+    n += (j.used && !j.finished) ? 1 : 0;
+  return n;
+}
+
+extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host, int layout, int nStepmax,
+                             double tEnd, int* job_id)
+{
+  if (!p || !jp || !host || !job_id)
+    return pfail(p, EK_EINVAL, "null argument");
+  std::string why;
+  if (ek::validate_params(jp, why))
+    return pfail(p, EK_EINVAL, why);
+  if (jp->dim != p->kind.dim || jp->mhd != p->kind.mhd || jp->riemann != p->kind.riemann ||
+      jp->slope_type != p->kind.slope_type || jp->precision != p->kind.precision)
+    return pfail(p, EK_EINVAL, "job does not belong to the pool's kind (dim, mhd, riemann, slope_type, precision)");
+  int id = -1;
+  for (int i = 0; i < p->max_jobs; i++)
+    if (!p->jobs[i].used) {
+      id = i;
+      break;
+    }
+  if (id < 0)
+    return pfail(p, EK_EFULL, "no free job slot");
+  const long long ncell = job_cells(*jp), nelem = ncell * jp->nbvar, need = round_up(nelem);
+  const long long off = arena_alloc(p, need);
+  if (off < 0)
+    return pfail(p, EK_EFULL, "arena full");
+  HostJob& h = p->jobs[id];
+  h = HostJob{};
+  h.used = true;
+  h.p = *jp;
+  h.off = off;
+  h.nelem = nelem;
+  h.alloc = need;
+  h.ncell = ncell;
+  h.adm_step = p->nstep;
+  h.nStepmax = nStepmax;
+  h.tEnd = tEnd;
+  const int par = (int)(p->nstep % 2);
+  char* dst = (char*)p->A[par] + (size_t)off * p->elem;
+  int rc = EK_OK;
+  if (layout == EK_LAYOUT_SOA) {
+    if (p->fp64) {
+      PHIP(p, hipMemcpyAsync(dst, host, (size_t)nelem * 8, hipMemcpyHostToDevice, p->stream));
+      PHIP(p, hipStreamSynchronize(p->stream));
+    } else {
+      std::vector<float> tmp((size_t)nelem);
+      for (long long i = 0; i < nelem; i++)
+        tmp[i] = (float)host[i];
+      PHIP(p, hipMemcpyAsync(dst, tmp.data(), (size_t)nelem * 4, hipMemcpyHostToDevice, p->stream));
+      PHIP(p, hipStreamSynchronize(p->stream));
+    }
+  } else if (layout == EK_LAYOUT_REFERENCE) {
+    const int is = jp->nx + 2 * jp->gw, js = jp->ny + 2 * jp->gw, ks = jp->nz + 2 * jp->gw;
+    const long long nr = (long long)is * js * (jp->dim == 3 ? ks : 1) * jp->nbvar;
+    if ((rc = ensure_stage(p, nr)))
+      return rc;
+    PHIP(p, hipMemcpyAsync(p->stage, host, (size_t)nr * 8, hipMemcpyHostToDevice, p->stream));
+    if (p->fp64) {
+      ek::Geom<double> g;
+      ek::Gas<double> gs;
+      ek::fill_geom<double>(*jp, g, gs);
+      PHIP(p, ek::fp64::ref_to_soa(jp->dim, p->stage, (double*)dst, g, jp->gw, jp->nbvar, is, js, ks, p->stream));
+    } else {
+      ek::Geom<float> g;
+      ek::Gas<float> gs;
+      ek::fill_geom<float>(*jp, g, gs);
+      PHIP(p, ek::fp32::ref_to_soa(jp->dim, p->stage, (float*)dst, g, jp->gw, jp->nbvar, is, js, ks, p->stream));
+    }
+    PHIP(p, hipStreamSynchronize(p->stream));
+  } else {
+    h.used = false;
+    arena_free(p, off, need);
+    return pfail(p, EK_EINVAL, "unknown layout");
+  }
+  // time slots: t = 0, it = 0 in the slot the next pool step reads, then its invDt
+  ek::TimeSlot init[3];
+  std::memset(init, 0, sizeof(init));
+  PHIP(p, hipMemcpyAsync(p->ts + 3 * id, init, sizeof(init), hipMemcpyHostToDevice, p->stream));
+  unsigned long long* bits = &p->ts[3 * id + (int)(p->nstep % 3)].invdt_bits;
+  if (p->fp64) {
+    ek::Geom<double> g;
+    ek::Gas<double> gs;
+    ek::fill_geom<double>(*jp, g, gs);
+    PHIP(p, ek::fp64::inv_dt(jp->dim, (const double*)dst, g, gs, bits, p->stream));
+  } else {
+    ek::Geom<float> g;
+    ek::Gas<float> gs;
+    ek::fill_geom<float>(*jp, g, gs);
+    PHIP(p, ek::fp32::inv_dt(jp->dim, (const float*)dst, g, gs, bits, p->stream));
+  }
+  if ((rc = upload_record(p, id)))
+    return rc;
+  if ((rc = rebuild_work(p)))
+    return rc;
+  *job_id = id;
+  return EK_OK;
+}
+
+extern "C" int ek_pool_step(ek_pool* p, int nsteps)
+{
+  if (!p || nsteps < 0)
+    return pfail(p, EK_EINVAL, "invalid argument");
+  int launches = 0;
+  PHIP(p, hipEventRecord(p->ev0, p->stream));
+  int left = nsteps;
+  while (left > 0) {
+    if (p->nstep % kGraphSteps == 0 && left >= kGraphSteps && p->nwork > 0) {
+      if (!p->graph_ok)
+        if (int rc = capture_graph(p))
+          return rc;
+      PHIP(p, hipGraphLaunch(p->graph, p->stream));
+      p->nstep += kGraphSteps;
+      left -= kGraphSteps;
+      launches += kGraphSteps;
+    } else {
+      PHIP(p, launch_one(p, p->nstep));
+      p->nstep += 1;
+      left -= 1;
+      launches += 1;
+    }
+  }
+  PHIP(p, hipEventRecord(p->ev1, p->stream));
+  PHIP(p, hipStreamSynchronize(p->stream));
+  float ms = 0.f;
+  PHIP(p, hipEventElapsedTime(&ms, p->ev0, p->ev1));
+  p->last_ms = ms;
+  p->last_launches = launches;
+  bool changed = false;
+  if (int rc = refresh_status(p, changed))
+    return rc;
+  if (changed)
+    return rebuild_work(p); // drop the tiles of jobs that finished
+  return EK_OK;
+}
+
+extern "C" int ek_pool_status(const ek_pool* p, int id, double* t, int* it, int* finished)
+{
+  if (!p || id < 0 || id >= p->max_jobs || !p->jobs[id].used)
+    return EK_EINVAL;
+  if (t)
+    *t = p->jobs[id].t;
+  if (it)
+    *it = p->jobs[id].it;
+  if (finished)
+    *finished = p->jobs[id].finished;
+  return EK_OK;
+}
+
+extern "C" int ek_pool_download(ek_pool* p, int id, double* host, int layout)
+{
+  if (!p || !host || id < 0 || id >= p->max_jobs || !p->jobs[id].used)
+    return pfail(p, EK_EINVAL, "invalid job");
+  const HostJob& h = p->jobs[id];
+  const char* src = (const char*)p->A[job_parity(p, h)] + (size_t)h.off * p->elem;
+  if (layout == EK_LAYOUT_SOA) {
+    if (p->fp64) {
+      PHIP(p, hipMemcpyAsync(host, src, (size_t)h.nelem * 8, hipMemcpyDeviceToHost, p->stream));
+      PHIP(p, hipStreamSynchronize(p->stream));
+    } else {
+      std::vector<float> tmp((size_t)h.nelem);
+      PHIP(p, hipMemcpyAsync(tmp.data(), src, (size_t)h.nelem * 4, hipMemcpyDeviceToHost, p->stream));
+      PHIP(p, hipStreamSynchronize(p->stream));
+      for (long long i = 0; i < h.nelem; i++)
+        host[i] = tmp[i];
+    }
+    return EK_OK;
+  }
+  if (layout != EK_LAYOUT_REFERENCE)
+    return pfail(p, EK_EINVAL, "unknown layout");
+  const ek_params& jp = h.p;
+  const int is = jp.nx + 2 * jp.gw, js = jp.ny + 2 * jp.gw, ks = jp.nz + 2 * jp.gw;
+  const long long nr = (long long)is * js * (jp.dim == 3 ? ks : 1) * jp.nbvar;
+  if (int rc = ensure_stage(p, nr))
+    return rc;
+  if (p->fp64) {
+    ek::Geom<double> g;
+    ek::Gas<double> gs;
+    ek::fill_geom<double>(jp, g, gs);
+    PHIP(p, ek::fp64::soa_to_ref(jp.dim, (const double*)src, p->stage, g, jp.gw, jp.nbvar, is, js, ks, jp.mhd,
+                                 p->stream));
+  } else {
+    ek::Geom<float> g;
+    ek::Gas<float> gs;
+    ek::fill_geom<float>(jp, g, gs);
+    PHIP(p, ek::fp32::soa_to_ref(jp.dim, (const float*)src, p->stage, g, jp.gw, jp.nbvar, is, js, ks, jp.mhd,
+                                 p->stream));
+  }
+  PHIP(p, hipMemcpyAsync(host, p->stage, (size_t)nr * 8, hipMemcpyDeviceToHost, p->stream));
+  PHIP(p, hipStreamSynchronize(p->stream));
+  return EK_OK;
+}
+
+extern "C" int ek_pool_budgets(ek_pool* p, int id, double* sums)
+{
+  if (!p || !sums || id < 0 || id >= p->max_jobs || !p->jobs[id].used)
+    return pfail(p, EK_EINVAL, "invalid job");
+  const HostJob& h = p->jobs[id];
+  const int nv = h.p.nbvar;
+  const int nb = (int)std::min<long long>(1024, (h.ncell + 255) / 256);
+  const char* src = (const char*)p->A[job_parity(p, h)] + (size_t)h.off * p->elem;
+  if (p->fp64)
+    PHIP(p, ek::fp64::budget(nv, (const double*)src, h.ncell, h.ncell, p->partial, nb, p->stream));
+  else
+    PHIP(p, ek::fp32::budget(nv, (const float*)src, h.ncell, h.ncell, p->partial, nb, p->stream));
+  std::vector<double> part((size_t)nb * nv * 2);
+  PHIP(p, hipMemcpyAsync(part.data(), p->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  PHIP(p, hipStreamSynchronize(p->stream));
+  for (int v = 0; v < nv; v++) {
+    double S = 0.0, C = 0.0;
+    for (int b = 0; b < nb; b++)
+      for (int k = 0; k < 2; k++) {
+        const double x = part[((size_t)b * nv + v) * 2 + k];
+        const double t = S + x;
+        if (std::fabs(S) >= std::fabs(x))
+          C += (S - t) + x;
+        else
+          C += (x - t) + S;
+        S = t;
+      }
+    sums[v] = S + C;
+  }
+  return EK_OK;
+}
+
+extern "C" int ek_pool_retire(ek_pool* p, int id)
+{
+  if (!p || id < 0 || id >= p->max_jobs || !p->jobs[id].used)
+    return pfail(p, EK_EINVAL, "invalid job");
+  HostJob& h = p->jobs[id];
+  arena_free(p, h.off, h.alloc);
+  h = HostJob{};
+  return rebuild_work(p);
+}
+
+static int apply_variant(ek_pool* p, int v)
+{
+  const int nvar = p->kind.dim == 2 ? ek::pool::N2VARIANTS : 4;
+  if (v < 0 || v >= nvar)
+    return pfail(p, EK_EINVAL, "tile choice out of range");
+  p->variant = v;
+  set_grid(p);
+  for (int id = 0; id < p->max_jobs; id++)
+    if (p->jobs[id].used)
+      if (int rc = upload_record(p, id))
+        return rc;
+  return rebuild_work(p);
+}
+
+extern "C" int ek_pool_set_tile(ek_pool* p, int choice)
+{
+  if (!p)
+    return EK_EINVAL;
+  return apply_variant(p, choice);
+}
+
+extern "C" int ek_pool_autotune(ek_pool* p, int steps, int* choice, double* ms_out, int n_ms)
+{
+  if (!p || steps <= 0)
+    return pfail(p, EK_EINVAL, "invalid argument");
+  const int nvar = p->kind.dim == 2 ? ek::pool::N2VARIANTS : 4;
+  // snapshot of the current buffer and the time slots, restored after every trial
+  long long hi = 0;
+  for (const HostJob& h : p->jobs)
+    if (h.used)
+      hi = std::max(hi, h.off + h.alloc);
+  void* snap = nullptr;
+  ek::TimeSlot* tsnap = nullptr;
+  const size_t sbytes = (size_t)hi * p->elem, tbytes = (size_t)p->max_jobs * 3 * sizeof(ek::TimeSlot);
+  if (hipMalloc(&snap, sbytes ? sbytes : 8) != hipSuccess || hipMalloc(&tsnap, tbytes) != hipSuccess) {
+    hipFree(snap);
+    return pfail(p, EK_ENOMEM, "autotune snapshot allocation failed");
+  }
+  const int par = (int)(p->nstep % 2);
+  int rc = EK_OK;
+  double best = 1e300;
+  int bestv = p->variant;
+  auto restore = [&]() -> hipError_t {
+    hipError_t e = hipMemcpyAsync(p->A[par], snap, sbytes, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(p->ts, tsnap, tbytes, hipMemcpyDeviceToDevice, p->stream);
+    if (e == hipSuccess)
+      e = hipStreamSynchronize(p->stream);
+    return e;
+  };
+  hipError_t e = hipMemcpyAsync(snap, p->A[par], sbytes, hipMemcpyDeviceToDevice, p->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(tsnap, p->ts, tbytes, hipMemcpyDeviceToDevice, p->stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(p->stream);
+  for (int v = 0; v < nvar && e == hipSuccess && rc == EK_OK; v++) {
+    if ((rc = apply_variant(p, v)))
+      break;
+    // one untimed step (first use of the kernel), restore, then the timed steps
+    if ((e = launch_one(p, p->nstep)) != hipSuccess || (e = restore()) != hipSuccess)
+      break;
+    if ((e = hipEventRecord(p->ev0, p->stream)) != hipSuccess)
+      break;
+    for (int s = 0; s < steps && e == hipSuccess; s++)
+      e = launch_one(p, p->nstep + s);
+    if (e != hipSuccess || (e = hipEventRecord(p->ev1, p->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(p->stream)) != hipSuccess)
+      break;
+    float ms = 0.f;
+    if ((e = hipEventElapsedTime(&ms, p->ev0, p->ev1)) != hipSuccess)
+      break;
+    const double per = ms / steps;
+    if (ms_out && v < n_ms)
+      ms_out[v] = per;
+    if (per < best) {
+      best = per;
+      bestv = v;
+    }
+    if ((e = restore()) != hipSuccess)
+      break;
+  }
+  hipFree(snap);
+  hipFree(tsnap);
+  if (e != hipSuccess)
+    return pfail(p, EK_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+  if (rc)
+    return rc;
+  if ((rc = apply_variant(p, bestv)))
+    return rc;
+  if (choice)
+    *choice = bestv;
+  return EK_OK;
+}
+
+extern "C" int ek_pool_last_step_time(const ek_pool* p, double* ms, int* launches)
+{
+  if (!p || !ms || !launches)
+    return EK_EINVAL;
+  *ms = p->last_ms;
+  *launches = p->last_launches;
+  return EK_OK;
+}

@@ -1,0 +1,114 @@
+"""The job pool on the GPU: every job's result equals its single-grid run (FP64 and FP32,
+bitwise), whether it runs alone or in a mix, is admitted at once or while others are
+already stepping; arena accounting; the autotuner leaves the jobs' states unchanged."""
+import numpy as np
+import pytest
+
+from euler_kokkos_amd import native, pool, problems
+from euler_kokkos_amd.params import make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def single_run(p, U0, nsteps, tEnd=1e30):
+    ctx = native.Context(p)
+    try:
+        ctx.upload(U0, native.EK_LAYOUT_SOA)
+        t, it, _ = ctx.run(nsteps, nsteps, tEnd)
+        return ctx.download(native.EK_LAYOUT_SOA), t, it
+    finally:
+        ctx.close()
+
+
+def jobs2d(precision):
+    out = []
+    for n, (nx, ny, bc, rad) in enumerate([(64, 64, (1,) * 4, "0.1"), (96, 40, (3, 3, 1, 2), "0.2"),
+                                           (33, 77, (2, 1, 3, 3), "0.15"), (128, 128, (1,) * 4, "0.05")]):
+        p = make_params("Hydro_Muscl_2D", nx, ny, gamma0="1.4", cfl="0.5", slope_type="2",
+                        riemann="hllc", bc=bc + (1, 1), precision=precision)
+        U = problems.blast(p, radius=rad, center_x="0.4", center_y="0.55", density_out="1",
+                           pressure_in=str(10 ** (1 + n)), pressure_out="0.1")
+        out.append((p, U))
+    return out
+
+
+@pytest.mark.parametrize("precision", [native.EK_FP64, native.EK_FP32])
+def test_pool_2d_matches_single_grid_with_staggered_admission(precision):
+    jobs = jobs2d(precision)
+    steps = [20, 13, 27, 9]
+    ref = [single_run(p, U, s) for (p, U), s in zip(jobs, steps)]
+    P = pool.Pool(jobs[0][0], 256 << 20, 16)
+    try:
+        ids = [P.admit(jobs[0][0], jobs[0][1], steps[0]), P.admit(jobs[1][0], jobs[1][1], steps[1])]
+        P.step(5)       # the first two run alone for a while
+        ids.append(P.admit(jobs[2][0], jobs[2][1], steps[2]))
+        P.step(7)
+        ids.append(P.admit(jobs[3][0], jobs[3][1], steps[3]))
+        P.step(40)      # everybody reaches nStepmax
+        for k, jid in enumerate(ids):
+            t, it, fin = P.status(jid)
+            G = P.download(jid)
+            print(f"case=pool2d prec={precision} job={k}: it={it} max|diff|={np.abs(G - ref[k][0]).max():.3e}",
+                  flush=True)
+            assert fin and it == steps[k] and it == ref[k][2] and t == ref[k][1]
+            assert np.array_equal(G, ref[k][0])
+    finally:
+        P.close()
+
+
+def test_pool_3d_matches_single_grid():
+    specs = [(32, 24, 20, (3,) * 6), (40, 32, 48, (1, 2, 3, 3, 1, 1))]
+    jobs = []
+    for nx, ny, nz, bc in specs:
+        p = make_params("Hydro_Muscl_3D", nx, ny, nz, gamma0="1.4", cfl="0.5", slope_type="2",
+                        riemann="hllc", bc=bc)
+        jobs.append((p, problems.blast(p, radius="0.2", density_out="1", pressure_out="0.1")))
+    ref = [single_run(p, U, 8) for p, U in jobs]
+    P = pool.Pool(jobs[0][0], 256 << 20, 8)
+    try:
+        ids = [P.admit(p, U, 8) for p, U in jobs]
+        P.step(8)
+        for k, jid in enumerate(ids):
+            assert np.array_equal(P.download(jid), ref[k][0])
+    finally:
+        P.close()
+
+
+def test_pool_arena_full_and_retire():
+    p = make_params("Hydro_Muscl_2D", 128, 128, gamma0="1.4", cfl="0.5", slope_type="2", riemann="hllc",
+                    precision=native.EK_FP32)
+    U = problems.blast(p, density_out="1")
+    P = pool.Pool(p, 3 * (2 * 128 * 128 * 4 * 4) + 4096, 8)
+    try:
+        need = P.job_bytes(p)
+        a = [P.admit(p, U, 4) for _ in range(3)]
+        with pytest.raises(pool.PoolFull):
+            P.admit(p, U, 4)
+        P.retire(a[1])
+        assert P.free_bytes() >= need
+        a[1] = P.admit(p, U, 4)
+        P.step(4)
+        ref = single_run(p, U, 4)[0]
+        for jid in a:
+            assert np.array_equal(P.download(jid), ref)
+    finally:
+        P.close()
+
+
+def test_autotune_restores_state_and_keeps_results():
+    jobs = jobs2d(native.EK_FP64)
+    ref = [single_run(p, U, 12) for p, U in jobs]
+    P = pool.Pool(jobs[0][0], 256 << 20, 8)
+    try:
+        ids = [P.admit(p, U, 12) for p, U in jobs]
+        P.step(3)
+        choice, ms = P.autotune(2)
+        print(f"case=autotune choice={choice} ms={ms}", flush=True)
+        assert 0 <= choice < 4 and all(m > 0 for m in ms)
+        P.step(9)
+        for k, jid in enumerate(ids):
+            assert np.array_equal(P.download(jid), ref[k][0])
+            b = P.budgets(jid)
+            assert abs(b[0] - ref[k][0][0].sum()) <= 1e-10 * ref[k][0][0].sum()
+    finally:
+        P.close()
