@@ -221,6 +221,40 @@ def batch_c4(n_runs, steps, seed):
     return res
 
 
+def serving_c5(n_jobs, rate, seed, steps):
+    """Serving stream: seeded Poisson arrivals of 3D jobs (edges uniform in [64,192], `steps`
+    steps each), admitted into device-resident FP32 job pools by HBM footprint and advanced
+    together (continuous batching); beside it the plain FP64 path serving the same stream
+    one job at a time."""
+    from euler_kokkos_amd import native, serving
+    res = {"workload": f"{n_jobs} 3D jobs, Poisson arrivals (mean {rate:g} jobs/s, seed {seed}), edges "
+                       f"uniform in [64,192], {steps} steps each; kinds: hydro Kelvin-Helmholtz shear layer "
+                       f"(HLLC + MC, gamma 1.4, CFL 0.5, periodic)",
+           "kinds": ["hydro_kh"]}
+    jobs = serving.poisson_stream(n_jobs, rate, seed, steps=steps, precision=native.EK_FP32)
+    for name, burst in (("stream", False), ("burst", True)):
+        # stream: the Poisson arrival times (latency at the offered load); burst: every job
+        # arrives at t = 0 (throughput at saturation)
+        arr = [j.arrival for j in jobs]
+        if burst:
+            for j in jobs:
+                j.arrival = 0.0
+        S = serving.Server(48 << 30, native.EK_FP32)
+        try:
+            fp32 = S.serve(jobs)
+        finally:
+            S.close()
+        for j in jobs:
+            j.done_t = -1.0
+        fp64 = serving.serve_one_at_a_time(jobs, native.EK_FP64)
+        for j, a in zip(jobs, arr):
+            j.arrival = a
+            j.done_t = -1.0
+        res[name] = {"fp32_continuous_batching": fp32, "fp64_one_at_a_time": fp64,
+                     "fp32_batched_vs_fp64_one_at_a_time": fp32["cell_updates_per_s"] / fp64["cell_updates_per_s"]}
+    return res
+
+
 def copy_calibration(gib=1.0):
     """A device-to-device copy of known size, for scaling rocprofv3 FETCH/WRITE counters."""
     hip = C.CDLL("libamdhip64.so")
@@ -253,6 +287,10 @@ def main() -> int:
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-seed", type=int, default=2024)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream line")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-seed", type=int, default=77)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build has no multi-GPU "
@@ -317,6 +355,8 @@ def main() -> int:
         out["mhd2d"] = mhd_c3(a.mhd_steps, not a.no_cpu, a.cpu_seconds / 2)
     if not a.no_batch:
         out["batch2d"] = batch_c4(a.batch_runs, a.batch_steps, a.batch_seed)
+    if not a.no_serve:
+        out["serving3d"] = serving_c5(a.serve_jobs, a.serve_rate, a.serve_seed, 50)
     print(json.dumps(out))
     return 0
 

@@ -1,0 +1,180 @@
+"""Serving: a request loop that admits simulation jobs into device-resident job pools by
+their HBM footprint and advances every admitted job together (continuous batching).
+
+Jobs arrive on a seeded Poisson process. At every loop turn the server admits, in arrival
+order, each job that has arrived and fits in its pool's arena; then every pool with work
+advances all of its jobs by one step in one launch; finished jobs are collected and
+retired, which frees memory for the queue. Time-to-solution is measured from a job's
+arrival to its completion; throughput over the whole stream.
+
+A job's result does not depend on what else runs with it: the pool steps each job with
+the per-tile code of its single-grid run (see tests/test_serving_gpu.py).
+"""
+from __future__ import annotations
+
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import native, problems
+from .params import make_params
+from .pool import MT19937_64, Pool, PoolFull
+
+
+@dataclass
+class Job:
+    id: int
+    kind: str          # "hydro_kh" (3D hydro Kelvin-Helmholtz shear layer)
+    n: int             # grid edge (n^3 cells)
+    steps: int
+    arrival: float     # seconds after the stream starts
+    seed: int
+    params: object = None
+    U0: np.ndarray = None
+    admit_t: float = -1.0
+    done_t: float = -1.0
+    result: np.ndarray = field(default=None, repr=False)
+
+    @property
+    def cells(self) -> int:
+        return self.n ** 3
+
+
+def kh3d(p, seed: int) -> np.ndarray:
+    """3D Kelvin-Helmholtz shear layer on the unit cube: rho = 2, vx = +0.5 where
+    |z - 1/2| < 1/4, rho = 1, vx = -0.5 elsewhere; vy = 0.01 sin(2 pi x) plus uniform
+    noise of amplitude 1e-3 (numpy PCG64 seeded with `seed`); vz = 0; p = 2.5."""
+    x = p.xmin + p.dx / 2 + np.arange(p.nx) * p.dx
+    z = p.zmin + p.dz / 2 + np.arange(p.nz) * p.dz
+    inner = (np.abs(z - 0.5) < 0.25)[:, None, None]
+    rng = np.random.default_rng(seed)
+    noise = rng.uniform(-1e-3, 1e-3, size=(p.nz, p.ny, p.nx))
+    rho = np.where(inner, 2.0, 1.0) * np.ones((p.nz, p.ny, p.nx))
+    vx = np.where(inner, 0.5, -0.5) * np.ones((p.nz, p.ny, p.nx))
+    vy = 0.01 * np.sin(2 * np.pi * x)[None, None, :] + noise
+    pr = 2.5
+    U = np.zeros((5, p.nz, p.ny, p.nx))
+    U[0] = rho
+    U[2] = rho * vx
+    U[3] = rho * vy
+    U[1] = pr / (p.gamma0 - 1.0) + 0.5 * rho * (vx * vx + vy * vy)
+    return U
+
+
+def job_params(kind: str, n: int, precision: int):
+    if kind == "hydro_kh":
+        return make_params("Hydro_Muscl_3D", n, n, n, gamma0="1.4", cfl="0.5", slope_type="2",
+                           riemann="hllc", bc=(3,) * 6, precision=precision)
+    raise ValueError(f"unknown job kind {kind}")
+
+
+def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh",), edge=(64, 192), steps=50,
+                   precision: int = native.EK_FP32, build_state: bool = True):
+    """Seeded job stream: exponential inter-arrival times of mean 1/rate seconds, kind
+    uniform over `kinds`, edge uniform over the integers [edge[0], edge[1]]."""
+    rng = MT19937_64(seed)
+    t = 0.0
+    jobs = []
+    for i in range(n_jobs):
+        t += -math.log(1.0 - rng.uniform()) / rate
+        kind = kinds[min(len(kinds) - 1, int(rng.uniform() * len(kinds)))]
+        n = edge[0] + min(edge[1] - edge[0], int(rng.uniform() * (edge[1] - edge[0] + 1)))
+        js = rng.next_u64() & 0x7FFFFFFF
+        j = Job(i, kind, n, steps, t, js)
+        j.params = job_params(kind, n, precision)
+        if build_state:
+            j.U0 = kh3d(j.params, js)
+        jobs.append(j)
+    return jobs
+
+
+class Server:
+    """Pools per job kind sharing an HBM budget; continuous batching over arrivals."""
+
+    def __init__(self, hbm_bytes: int, precision: int = native.EK_FP32, kinds=("hydro_kh",), max_jobs=512):
+        self.pools = {}
+        share = hbm_bytes // len(kinds)
+        for k in kinds:
+            self.pools[k] = Pool(job_params(k, 64, precision), share, max_jobs)
+        self.precision = precision
+        self.step_s = 0.0  # wall time of one pool step (all busy pools), last measured
+
+    def close(self):
+        for p in self.pools.values():
+            p.close()
+
+    def serve(self, jobs, keep_results: bool = False, clock=time.perf_counter, sleep=time.sleep):
+        pending = deque(sorted(jobs, key=lambda j: j.arrival))
+        active = {}  # (kind, jid) -> Job
+        t0 = clock()
+        launches = 0
+        while pending or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                j = pending[0]
+                try:
+                    jid = self.pools[j.kind].admit(j.params, j.U0, j.steps)
+                except PoolFull:
+                    break  # first come, first served: wait for memory
+                j.admit_t = clock() - t0
+                active[(j.kind, jid)] = j
+                pending.popleft()
+            if not active:
+                if pending:
+                    sleep(max(0.0, pending[0].arrival - (clock() - t0)))
+                continue
+            # advance to the earliest completion (at most a graph's worth of steps), but do
+            # not run past the next arrival by more than one step
+            busy = {k for k, _ in active}
+            nstep = max(1, min(6, min(j.steps - self.pools[k].status(jid)[1] for (k, jid), j in active.items())))
+            if pending and self.step_s > 0.0:
+                gap = pending[0].arrival - (clock() - t0)
+                nstep = max(1, min(nstep, int(gap / self.step_s)))
+            ts = clock()
+            for k in busy:
+                self.pools[k].step(nstep)
+                launches += nstep
+            self.step_s = (clock() - ts) / nstep
+            done_t = clock() - t0
+            for key in list(active):
+                k, jid = key
+                _, it, fin = self.pools[k].status(jid)
+                if fin:
+                    j = active.pop(key)
+                    if keep_results:
+                        j.result = self.pools[k].download(jid)
+                    self.pools[k].retire(jid)
+                    j.done_t = done_t
+        return summarize(jobs, launches)
+
+
+def summarize(jobs, launches=0):
+    tts = np.array([j.done_t - j.arrival for j in jobs])
+    first = min(j.arrival for j in jobs)
+    last = max(j.done_t for j in jobs)
+    cells = sum(j.cells * j.steps for j in jobs)
+    return {"jobs": len(jobs), "jobs_per_s": len(jobs) / (last - first), "cell_updates_per_s": cells / (last - first),
+            "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
+            "tts_max_s": float(tts.max()), "span_s": last - first, "pool_launches": launches}
+
+
+def serve_one_at_a_time(jobs, precision=native.EK_FP64, clock=time.perf_counter, sleep=time.sleep):
+    """The plain path: one job at a time in arrival order, each in its own context (ek_run)."""
+    t0 = clock()
+    for j in sorted(jobs, key=lambda j: j.arrival):
+        now = clock() - t0
+        if j.arrival > now:
+            sleep(j.arrival - now)
+        p = job_params(j.kind, j.n, precision)
+        ctx = native.Context(p)
+        try:
+            ctx.upload(j.U0, native.EK_LAYOUT_SOA)
+            ctx.run(j.steps, j.steps, 1e30)
+            ctx.sync()
+        finally:
+            ctx.close()
+        j.done_t = clock() - t0
+    return summarize(jobs)

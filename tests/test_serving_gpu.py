@@ -1,0 +1,55 @@
+"""Continuous batching on the GPU: each job's result is the same alone and in the mix
+(bitwise: the pool steps every job with its single-grid per-tile code), jobs wait for HBM
+when the arena is full, and time-to-solution is measured from arrival."""
+import numpy as np
+import pytest
+
+from euler_kokkos_amd import native, serving
+from euler_kokkos_amd.pool import Pool
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(seed=5, edge=(20, 40)):
+    return serving.poisson_stream(8, rate=400.0, seed=seed, edge=edge, steps=7)
+
+
+def alone(j):
+    P = Pool(j.params, 64 << 20, 4)
+    try:
+        jid = P.admit(j.params, j.U0, j.steps)
+        P.step(j.steps + 1)
+        assert P.status(jid)[2]
+        return P.download(jid)
+    finally:
+        P.close()
+
+
+def test_each_result_same_alone_and_in_the_mix():
+    jobs = small_stream()
+    S = serving.Server(256 << 20)
+    try:
+        stats = S.serve(jobs, keep_results=True)
+    finally:
+        S.close()
+    print(f"case=serving mix: {stats}", flush=True)
+    assert stats["jobs"] == 8
+    for j in jobs:
+        assert j.done_t >= j.arrival and j.result is not None
+        ref = alone(j)
+        assert np.array_equal(j.result, ref), j.id
+
+
+def test_admission_waits_for_memory():
+    jobs = small_stream(seed=9, edge=(32, 32))
+    biggest = max(2 * j.cells * 5 * 4 for j in jobs)
+    S = serving.Server(biggest + (1 << 16))  # room for one job at a time
+    try:
+        stats = S.serve(jobs, keep_results=True)
+    finally:
+        S.close()
+    spans = sorted((j.admit_t, j.done_t) for j in jobs)
+    for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
+        assert a1 >= d0 - 1e-9  # never two jobs resident together
+    for j in jobs:
+        assert np.array_equal(j.result, alone(j))
