@@ -104,6 +104,20 @@ void eko_mhd_update(const eko_params* p, double* U, const double* Fx, const doub
 int eko_mhd_run(const eko_params* p, double* U, int nStepmax, double tEnd, double* t_io, int* it_io,
                 double* dt_last);
 
+/* ---- 3D MHD (SolverMHDMuscl<3>, impl 0), in ek_oracle_mhd.c ---- */
+void eko_mhd3d_convert_to_primitives(const eko_params* p, const double* U, double* Q);
+double eko_mhd3d_inv_dt(const eko_params* p, const double* Q);
+void eko_mhd3d_elec_field(const eko_params* p, const double* U, const double* Q, double* E);
+void eko_mhd3d_mag_slopes(const eko_params* p, const double* U, double* dA, double* dB, double* dC);
+void eko_mhd3d_trace(const eko_params* p, const double* q, double dq[3][8], const double bfNb[6],
+                     const double dbf[12], const double E[3][2][2], double dtdx, double dtdy, double dtdz,
+                     double qm[3][8], double qp[3][8], double qE[4][3][8]);
+double eko_compute_emf_dir(const eko_params* p, double qEdge[4][8], int dir);
+int eko_mhd_run_dt(const eko_params* p, double* U, int nsteps, double dt);
+int eko_mhd3d_run_dt(const eko_params* p, double* U, int nsteps, double dt);
+int eko_mhd3d_run(const eko_params* p, double* U, int nStepmax, double tEnd, double* t_io, int* it_io,
+                  double* dt_last);
+
 /* Number of OpenMP threads the oracle uses (1 without OpenMP). */
 int eko_num_threads(void);
 

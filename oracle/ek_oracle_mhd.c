@@ -793,6 +793,48 @@ void eko_mhd_init_orszag_tang(const eko_params* p, double* U)
     }
 }
 
+/* nsteps 2D MHD steps with a fixed dt (the step of godunov_unsplit_impl after compute_dt) */
+int eko_mhd_run_dt(const eko_params* p, double* U, int nsteps, double dt)
+{
+  const long n = (long)p->isize * p->jsize * NV;
+  const long ns = (long)p->isize * p->jsize;
+  double* U2 = (double*)malloc(sizeof(double) * n);
+  double* Q = (double*)calloc(n, sizeof(double));
+  double* Fx = (double*)calloc(n, sizeof(double));
+  double* Fy = (double*)calloc(n, sizeof(double));
+  double* Emf = (double*)calloc(ns, sizeof(double));
+  double* tr[8];
+  int ok = U2 && Q && Fx && Fy && Emf;
+  for (int a = 0; a < 8; a++) {
+    tr[a] = (double*)calloc(n, sizeof(double));
+    ok = ok && tr[a];
+  }
+  if (!ok)
+    return 1;
+  for (int s = 0; s < nsteps; s++) {
+    double* in = (s % 2 == 0) ? U : U2;
+    double* out = (s % 2 == 0) ? U2 : U;
+    eko_make_boundaries(p, in);
+    memcpy(out, in, sizeof(double) * n);
+    eko_mhd_convert_to_primitives(p, in, Q);
+    const double dtdx = dt / p->dx, dtdy = dt / p->dy;
+    eko_mhd_compute_trace(p, in, Q, tr, dtdx, dtdy);
+    eko_mhd_compute_fluxes(p, tr, Fx, Fy);
+    eko_mhd_compute_emf(p, tr, Emf);
+    eko_mhd_update(p, out, Fx, Fy, Emf, dtdx, dtdy);
+  }
+  if (nsteps % 2 == 1)
+    memcpy(U, U2, sizeof(double) * n);
+  free(U2);
+  free(Q);
+  free(Fx);
+  free(Fy);
+  free(Emf);
+  for (int a = 0; a < 8; a++)
+    free(tr[a]);
+  return 0;
+}
+
 /* SolverMHDMuscl<2> time loop: next_iteration_impl + godunov_unsplit_impl (impl 0)
  * (src/muscl/SolverMHDMuscl.h:741-801, SolverMHDMuscl.cpp:369-454). dt is computed inside
  * the step from the primitives of the current state. */
@@ -849,5 +891,508 @@ int eko_mhd_run(const eko_params* p, double* U, int nStepmax, double tEnd, doubl
   free(Emf);
   for (int a = 0; a < 8; a++)
     free(tr[a]);
+  return 0;
+}
+
+/* ================================================================ 3D MHD (impl 0) */
+/* SolverMHDMuscl<3>::godunov_unsplit_impl (src/muscl/SolverMHDMuscl.cpp:462-512). Arrays
+ * on the reference layout U(i,j,k,var), ghosts included, gw = 3. */
+
+#define IX3(p, i, j, k, v) (((((long)(i)) * (p)->jsize + (j)) * (p)->ksize + (k)) * NV + (v))
+#define IV3(p, i, j, k, c) (((((long)(i)) * (p)->jsize + (j)) * (p)->ksize + (k)) * 3 + (c))
+
+/* ConvertToPrimitivesFunctor3D_MHD (MHDRunFunctors3D.h:88-163) */
+void eko_mhd3d_convert_to_primitives(const eko_params* p, const double* U, double* Q)
+{
+#pragma omp parallel for collapse(2)
+  for (int i = 0; i < p->isize - 1; i++)
+    for (int j = 0; j < p->jsize - 1; j++)
+      for (int k = 0; k < p->ksize - 1; k++) {
+        double u[NV], q[NV], mag[3], c;
+        for (int v = 0; v < NV; v++)
+          u[v] = U[IX3(p, i, j, k, v)];
+        mag[0] = U[IX3(p, i + 1, j, k, IA)];
+        mag[1] = U[IX3(p, i, j + 1, k, IB)];
+        mag[2] = U[IX3(p, i, j, k + 1, IC)];
+        eko_mhd_constoprim(p, u, mag, &c, q);
+        for (int v = 0; v < NV; v++)
+          Q[IX3(p, i, j, k, v)] = q[v];
+      }
+}
+
+/* ComputeDtFunctor3D_MHD (MHDRunFunctors3D.h:16-87) */
+double eko_mhd3d_inv_dt(const eko_params* p, const double* Q)
+{
+  const int gw = p->gw;
+  double invDt = 0.0;
+#pragma omp parallel for collapse(2) reduction(max : invDt)
+  for (int i = gw; i < p->isize - gw; i++)
+    for (int j = gw; j < p->jsize - gw; j++)
+      for (int k = gw; k < p->ksize - gw; k++) {
+        double q[NV], s[3];
+        for (int v = 0; v < NV; v++)
+          q[v] = Q[IX3(p, i, j, k, v)];
+        eko_params p1 = *p;
+        p1.dim = 3;
+        eko_mhd_speed_info(&p1, q, s);
+        invDt = fmax(invDt, s[0] / p->dx + s[1] / p->dy + s[2] / p->dz);
+      }
+  return invDt;
+}
+
+/* ComputeElecFieldFunctor3D (MHDRunFunctors3D.h:278-362): E(i,j,k,{IX,IY,IZ}) */
+void eko_mhd3d_elec_field(const eko_params* p, const double* U, const double* Q, double* E)
+{
+#pragma omp parallel for collapse(2)
+  for (int i = 1; i < p->isize - 1; i++)
+    for (int j = 1; j < p->jsize - 1; j++)
+      for (int k = 1; k < p->ksize - 1; k++) {
+        double u, v, w, A, B, C;
+        v = 0.25 * (Q[IX3(p, i, j - 1, k - 1, IV)] + Q[IX3(p, i, j - 1, k, IV)] + Q[IX3(p, i, j, k - 1, IV)] +
+                    Q[IX3(p, i, j, k, IV)]);
+        w = 0.25 * (Q[IX3(p, i, j - 1, k - 1, IW)] + Q[IX3(p, i, j - 1, k, IW)] + Q[IX3(p, i, j, k - 1, IW)] +
+                    Q[IX3(p, i, j, k, IW)]);
+        B = HALF * (U[IX3(p, i, j, k - 1, IB)] + U[IX3(p, i, j, k, IB)]);
+        C = HALF * (U[IX3(p, i, j - 1, k, IC)] + U[IX3(p, i, j, k, IC)]);
+        E[IV3(p, i, j, k, 0)] = v * C - w * B;
+        u = 0.25 * (Q[IX3(p, i - 1, j, k - 1, IU)] + Q[IX3(p, i - 1, j, k, IU)] + Q[IX3(p, i, j, k - 1, IU)] +
+                    Q[IX3(p, i, j, k, IU)]);
+        w = 0.25 * (Q[IX3(p, i - 1, j, k - 1, IW)] + Q[IX3(p, i - 1, j, k, IW)] + Q[IX3(p, i, j, k - 1, IW)] +
+                    Q[IX3(p, i, j, k, IW)]);
+        A = HALF * (U[IX3(p, i, j, k - 1, IA)] + U[IX3(p, i, j, k, IA)]);
+        C = HALF * (U[IX3(p, i - 1, j, k, IC)] + U[IX3(p, i, j, k, IC)]);
+        E[IV3(p, i, j, k, 1)] = w * A - u * C;
+        u = 0.25 * (Q[IX3(p, i - 1, j - 1, k, IU)] + Q[IX3(p, i - 1, j, k, IU)] + Q[IX3(p, i, j - 1, k, IU)] +
+                    Q[IX3(p, i, j, k, IU)]);
+        v = 0.25 * (Q[IX3(p, i - 1, j - 1, k, IV)] + Q[IX3(p, i - 1, j, k, IV)] + Q[IX3(p, i, j - 1, k, IV)] +
+                    Q[IX3(p, i, j, k, IV)]);
+        A = HALF * (U[IX3(p, i, j - 1, k, IA)] + U[IX3(p, i, j, k, IA)]);
+        B = HALF * (U[IX3(p, i - 1, j, k, IB)] + U[IX3(p, i, j, k, IB)]);
+        E[IV3(p, i, j, k, 2)] = u * B - v * A;
+      }
+}
+
+/* one limited transverse slope of slope_unsplit_mhd_3d (MHDBaseFunctor3D.h:563-668) */
+static double slope_bf3(double xst, double b, double bplus, double bminus)
+{
+  double dlft, drgt, dcen, dsgn, slop, dlim;
+  dlft = xst * (b - bminus);
+  drgt = xst * (bplus - b);
+  dcen = HALF * (bplus - bminus);
+  dsgn = (dcen >= ZERO) ? ONE : -ONE;
+  slop = fmin(fabs(dlft), fabs(drgt));
+  dlim = slop;
+  if ((dlft * drgt) <= ZERO)
+    dlim = ZERO;
+  return dsgn * fmin(dlim, fabs(dcen));
+}
+
+/* ComputeMagSlopesFunctor3D (MHDRunFunctors3D.h:441-538): dA[3], dB[3], dC[3] per cell
+ * (x, y, z slopes; the normal one is zero) */
+void eko_mhd3d_mag_slopes(const eko_params* p, const double* U, double* dA, double* dB, double* dC)
+{
+  const double xst = fmin(p->slope_type, 2.0);
+#pragma omp parallel for collapse(2)
+  for (int i = 1; i < p->isize - 1; i++)
+    for (int j = 1; j < p->jsize - 1; j++)
+      for (int k = 1; k < p->ksize - 1; k++) {
+        const double a = U[IX3(p, i, j, k, IA)], b = U[IX3(p, i, j, k, IB)], c = U[IX3(p, i, j, k, IC)];
+        dA[IV3(p, i, j, k, 0)] = 0.0;
+        dA[IV3(p, i, j, k, 1)] = slope_bf3(xst, a, U[IX3(p, i, j + 1, k, IA)], U[IX3(p, i, j - 1, k, IA)]);
+        dA[IV3(p, i, j, k, 2)] = slope_bf3(xst, a, U[IX3(p, i, j, k + 1, IA)], U[IX3(p, i, j, k - 1, IA)]);
+        dB[IV3(p, i, j, k, 0)] = slope_bf3(xst, b, U[IX3(p, i + 1, j, k, IB)], U[IX3(p, i - 1, j, k, IB)]);
+        dB[IV3(p, i, j, k, 1)] = 0.0;
+        dB[IV3(p, i, j, k, 2)] = slope_bf3(xst, b, U[IX3(p, i, j, k + 1, IB)], U[IX3(p, i, j, k - 1, IB)]);
+        dC[IV3(p, i, j, k, 0)] = slope_bf3(xst, c, U[IX3(p, i + 1, j, k, IC)], U[IX3(p, i - 1, j, k, IC)]);
+        dC[IV3(p, i, j, k, 1)] = slope_bf3(xst, c, U[IX3(p, i, j + 1, k, IC)], U[IX3(p, i, j - 1, k, IC)]);
+        dC[IV3(p, i, j, k, 2)] = 0.0;
+      }
+}
+
+/* trace_unsplit_mhd_3d_simpler (MHDBaseFunctor3D.h:690-1110). dq: half-step slopes are
+ * formed here from the full ones; out: qm[3], qp[3], edges [4][3] = (RT,RB,LT,LB) x (X,Y,Z). */
+void eko_mhd3d_trace(const eko_params* p, const double* q, double dq[3][NV], const double bfNb[6],
+                     const double dbf[12], const double E[3][2][2], double dtdx, double dtdy, double dtdz,
+                     double qm[3][NV], double qp[3][NV], double qE[4][3][NV])
+{
+  const double gamma = p->gamma0, smallR = p->smallr, smallp = p->smallp;
+  const double ELL = E[0][0][0], ELR = E[0][0][1], ERL = E[0][1][0], ERR = E[0][1][1];
+  const double FLL = E[1][0][0], FLR = E[1][0][1], FRL = E[1][1][0], FRR = E[1][1][1];
+  const double GLL = E[2][0][0], GLR = E[2][0][1], GRL = E[2][1][0], GRR = E[2][1][1];
+  double r = q[ID], pr = q[IP], u = q[IU], v = q[IV], w = q[IW], A = q[IA], B = q[IB], C = q[IC];
+  double AL = bfNb[0], AR = bfNb[1], BL = bfNb[2], BR = bfNb[3], CL = bfNb[4], CR = bfNb[5];
+  double drx = dq[0][ID] * HALF, dpx = dq[0][IP] * HALF, dux = dq[0][IU] * HALF, dvx = dq[0][IV] * HALF;
+  double dwx = dq[0][IW] * HALF, dCx = dq[0][IC] * HALF, dBx = dq[0][IB] * HALF;
+  double dry = dq[1][ID] * HALF, dpy = dq[1][IP] * HALF, duy = dq[1][IU] * HALF, dvy = dq[1][IV] * HALF;
+  double dwy = dq[1][IW] * HALF, dCy = dq[1][IC] * HALF, dAy = dq[1][IA] * HALF;
+  double drz = dq[2][ID] * HALF, dpz = dq[2][IP] * HALF, duz = dq[2][IU] * HALF, dvz = dq[2][IV] * HALF;
+  double dwz = dq[2][IW] * HALF, dAz = dq[2][IA] * HALF, dBz = dq[2][IB] * HALF;
+  double dALy = HALF * dbf[0], dALz = HALF * dbf[1], dBLx = HALF * dbf[2], dBLz = HALF * dbf[3];
+  double dCLx = HALF * dbf[4], dCLy = HALF * dbf[5];
+  double dARy = HALF * dbf[6], dARz = HALF * dbf[7], dBRx = HALF * dbf[8], dBRz = HALF * dbf[9];
+  double dCRx = HALF * dbf[10], dCRy = HALF * dbf[11];
+  double dAx = HALF * (AR - AL), dBy = HALF * (BR - BL), dCz = HALF * (CR - CL);
+  double sr0, su0, sv0, sw0, sp0, sA0, sB0, sC0, sAL0, sAR0, sBL0, sBR0, sCL0, sCR0;
+  sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy + (-w * drz - dwz * r) * dtdz;
+  su0 = (-u * dux - (dpx + B * dBx + C * dCx) / r) * dtdx + (-v * duy + B * dAy / r) * dtdy +
+        (-w * duz + C * dAz / r) * dtdz;
+  sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - (dpy + A * dAy + C * dCy) / r) * dtdy +
+        (-w * dvz + C * dBz / r) * dtdz;
+  sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy +
+        (-w * dwz - (dpz + A * dAz + B * dBz) / r) * dtdz;
+  sp0 = (-u * dpx - dux * gamma * pr) * dtdx + (-v * dpy - dvy * gamma * pr) * dtdy +
+        (-w * dpz - dwz * gamma * pr) * dtdz;
+  sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - w * dAz - A * dwz) * dtdz;
+  sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - w * dBz - B * dwz) * dtdz;
+  sC0 = (w * dAx + A * dwx - u * dCx - C * dux) * dtdx + (w * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+  sAL0 = +(GLR - GLL) * dtdy * HALF - (FLR - FLL) * dtdz * HALF;
+  sAR0 = +(GRR - GRL) * dtdy * HALF - (FRR - FRL) * dtdz * HALF;
+  sBL0 = -(GRL - GLL) * dtdx * HALF + (ELR - ELL) * dtdz * HALF;
+  sBR0 = -(GRR - GLR) * dtdx * HALF + (ERR - ERL) * dtdz * HALF;
+  sCL0 = +(FRL - FLL) * dtdx * HALF - (ERL - ELL) * dtdy * HALF;
+  sCR0 = +(FRR - FLR) * dtdx * HALF - (ERR - ELR) * dtdy * HALF;
+  r = r + sr0;
+  u = u + su0;
+  v = v + sv0;
+  w = w + sw0;
+  pr = pr + sp0;
+  A = A + sA0;
+  B = B + sB0;
+  C = C + sC0;
+  AL = AL + sAL0;
+  AR = AR + sAR0;
+  BL = BL + sBL0;
+  BR = BR + sBR0;
+  CL = CL + sCL0;
+  CR = CR + sCR0;
+#define FLOOR3(s)                    \
+  do {                               \
+    (s)[ID] = fmax(smallR, (s)[ID]); \
+    (s)[IP] = fmax(smallp, (s)[IP]); \
+  } while (0)
+#define SET8(s, a0, a1, a2, a3, a4, a5, a6, a7) \
+  do {                                          \
+    (s)[ID] = (a0);                             \
+    (s)[IU] = (a1);                             \
+    (s)[IV] = (a2);                             \
+    (s)[IW] = (a3);                             \
+    (s)[IP] = (a4);                             \
+    (s)[IA] = (a5);                             \
+    (s)[IB] = (a6);                             \
+    (s)[IC] = (a7);                             \
+    FLOOR3(s);                                  \
+  } while (0)
+  SET8(qp[0], r - drx, u - dux, v - dvx, w - dwx, pr - dpx, AL, B - dBx, C - dCx);
+  SET8(qm[0], r + drx, u + dux, v + dvx, w + dwx, pr + dpx, AR, B + dBx, C + dCx);
+  SET8(qp[1], r - dry, u - duy, v - dvy, w - dwy, pr - dpy, A - dAy, BL, C - dCy);
+  SET8(qm[1], r + dry, u + duy, v + dvy, w + dwy, pr + dpy, A + dAy, BR, C + dCy);
+  SET8(qp[2], r - drz, u - duz, v - dvz, w - dwz, pr - dpz, A - dAz, B - dBz, CL);
+  SET8(qm[2], r + drz, u + duz, v + dvz, w + dwz, pr + dpz, A + dAz, B + dBz, CR);
+  /* X edges */
+  SET8(qE[0][0], r + (+dry + drz), u + (+duy + duz), v + (+dvy + dvz), w + (+dwy + dwz), pr + (+dpy + dpz),
+       A + (+dAy + dAz), BR + (+dBRz), CR + (+dCRy));
+  SET8(qE[1][0], r + (+dry - drz), u + (+duy - duz), v + (+dvy - dvz), w + (+dwy - dwz), pr + (+dpy - dpz),
+       A + (+dAy - dAz), BR + (-dBRz), CL + (+dCLy));
+  SET8(qE[2][0], r + (-dry + drz), u + (-duy + duz), v + (-dvy + dvz), w + (-dwy + dwz), pr + (-dpy + dpz),
+       A + (-dAy + dAz), BL + (+dBLz), CR + (-dCRy));
+  SET8(qE[3][0], r + (-dry - drz), u + (-duy - duz), v + (-dvy - dvz), w + (-dwy - dwz), pr + (-dpy - dpz),
+       A + (-dAy - dAz), BL + (-dBLz), CL + (-dCLy));
+  /* Y edges */
+  SET8(qE[0][1], r + (+drx + drz), u + (+dux + duz), v + (+dvx + dvz), w + (+dwx + dwz), pr + (+dpx + dpz),
+       AR + (+dARz), B + (+dBx + dBz), CR + (+dCRx));
+  SET8(qE[1][1], r + (+drx - drz), u + (+dux - duz), v + (+dvx - dvz), w + (+dwx - dwz), pr + (+dpx - dpz),
+       AR + (-dARz), B + (+dBx - dBz), CL + (+dCLx));
+  SET8(qE[2][1], r + (-drx + drz), u + (-dux + duz), v + (-dvx + dvz), w + (-dwx + dwz), pr + (-dpx + dpz),
+       AL + (+dALz), B + (-dBx + dBz), CR + (-dCRx));
+  SET8(qE[3][1], r + (-drx - drz), u + (-dux - duz), v + (-dvx - dvz), w + (-dwx - dwz), pr + (-dpx - dpz),
+       AL + (-dALz), B + (-dBx - dBz), CL + (-dCLx));
+  /* Z edges */
+  SET8(qE[0][2], r + (+drx + dry), u + (+dux + duy), v + (+dvx + dvy), w + (+dwx + dwy), pr + (+dpx + dpy),
+       AR + (+dARy), BR + (+dBRx), C + (+dCx + dCy));
+  SET8(qE[1][2], r + (+drx - dry), u + (+dux - duy), v + (+dvx - dvy), w + (+dwx - dwy), pr + (+dpx - dpy),
+       AR + (-dARy), BL + (+dBLx), C + (+dCx - dCy));
+  SET8(qE[2][2], r + (-drx + dry), u + (-dux + duy), v + (-dvx + dvy), w + (-dwx + dwy), pr + (-dpx + dpy),
+       AL + (+dALy), BR + (-dBRx), C + (-dCx + dCy));
+  SET8(qE[3][2], r + (-drx - dry), u + (-dux - duy), v + (-dvx - dvy), w + (-dwx - dwy), pr + (-dpx - dpy),
+       AL + (-dALy), BL + (-dBLx), C + (-dCx - dCy));
+#undef SET8
+#undef FLOOR3
+}
+
+/* compute_emf<EMFY/EMFX> state mapping (RiemannSolvers_MHD.h:736-818): dir 0 = X, 1 = Y, 2 = Z */
+double eko_compute_emf_dir(const eko_params* p, double qEdge[4][NV], int dir)
+{
+  if (dir == 2)
+    return eko_compute_emfz(p, qEdge);
+  /* (u, v, A, B, w, c) components per direction */
+  const int cu = dir == 1 ? IW : IV, cv = dir == 1 ? IU : IW, ca = dir == 1 ? IC : IB, cb = dir == 1 ? IA : IC;
+  const int cw = dir == 1 ? IV : IU, cc = dir == 1 ? IB : IA;
+  double t[4][NV];
+  for (int e = 0; e < 4; e++) {
+    t[e][ID] = qEdge[e][ID];
+    t[e][IP] = qEdge[e][IP];
+    t[e][IU] = qEdge[e][cu];
+    t[e][IV] = qEdge[e][cv];
+    t[e][IA] = qEdge[e][ca];
+    t[e][IB] = qEdge[e][cb];
+    t[e][IW] = qEdge[e][cw];
+    t[e][IC] = qEdge[e][cc];
+  }
+  return eko_compute_emfz(p, t);
+}
+
+static void swap2(double* s, int a, int b)
+{
+  double x = s[a];
+  s[a] = s[b];
+  s[b] = x;
+}
+
+/* One 3D MHD step (godunov_unsplit_impl, impl 0) from `in` (ghosts already filled) into
+ * `out` (a copy of `in`), with the given dt. */
+static int mhd3d_step(const eko_params* p, const double* in, double* out, double dt)
+{
+  const long ncell = (long)p->isize * p->jsize * p->ksize;
+  const int gw = p->gw;
+  double* Q = (double*)calloc(ncell * NV, sizeof(double));
+  double* E = (double*)calloc(ncell * 3, sizeof(double));
+  double* dA = (double*)calloc(ncell * 3, sizeof(double));
+  double* dB = (double*)calloc(ncell * 3, sizeof(double));
+  double* dC = (double*)calloc(ncell * 3, sizeof(double));
+  double* F[3];
+  double* T[18]; /* qm x,y,z; qp x,y,z; edges (RT,RB,LT,LB) x (X,Y,Z) */
+  double* M = (double*)calloc(ncell * 3, sizeof(double)); /* EMF x, y, z */
+  int ok = Q && E && dA && dB && dC && M;
+  for (int a = 0; a < 3; a++) {
+    F[a] = (double*)calloc(ncell * NV, sizeof(double));
+    ok = ok && F[a];
+  }
+  for (int a = 0; a < 18; a++) {
+    T[a] = (double*)calloc(ncell * NV, sizeof(double));
+    ok = ok && T[a];
+  }
+  if (!ok)
+    return 1;
+  const double dtdx = dt / p->dx, dtdy = dt / p->dy, dtdz = dt / p->dz;
+  eko_mhd3d_convert_to_primitives(p, in, Q);
+  eko_mhd3d_elec_field(p, in, Q, E);
+  eko_mhd3d_mag_slopes(p, in, dA, dB, dC);
+  /* ComputeTraceFunctor3D_MHD (MHDRunFunctors3D.h:543-856) */
+#pragma omp parallel for collapse(2)
+  for (int i = gw - 2; i < p->isize - gw + 1; i++)
+    for (int j = gw - 2; j < p->jsize - gw + 1; j++)
+      for (int k = gw - 2; k < p->ksize - gw + 1; k++) {
+        double q[NV], dq[3][NV], bf[6], dbf[12], El[3][2][2], qm[3][NV], qp[3][NV], qE[4][3][NV];
+        for (int v = 0; v < NV; v++) {
+          q[v] = Q[IX3(p, i, j, k, v)];
+          dq[0][v] = slope_mhd(p, q[v], Q[IX3(p, i + 1, j, k, v)], Q[IX3(p, i - 1, j, k, v)]);
+          dq[1][v] = slope_mhd(p, q[v], Q[IX3(p, i, j + 1, k, v)], Q[IX3(p, i, j - 1, k, v)]);
+          dq[2][v] = slope_mhd(p, q[v], Q[IX3(p, i, j, k + 1, v)], Q[IX3(p, i, j, k - 1, v)]);
+        }
+        bf[0] = in[IX3(p, i, j, k, IA)];
+        bf[1] = in[IX3(p, i + 1, j, k, IA)];
+        bf[2] = in[IX3(p, i, j, k, IB)];
+        bf[3] = in[IX3(p, i, j + 1, k, IB)];
+        bf[4] = in[IX3(p, i, j, k, IC)];
+        bf[5] = in[IX3(p, i, j, k + 1, IC)];
+        dbf[0] = dA[IV3(p, i, j, k, 1)];
+        dbf[1] = dA[IV3(p, i, j, k, 2)];
+        dbf[2] = dB[IV3(p, i, j, k, 0)];
+        dbf[3] = dB[IV3(p, i, j, k, 2)];
+        dbf[4] = dC[IV3(p, i, j, k, 0)];
+        dbf[5] = dC[IV3(p, i, j, k, 1)];
+        dbf[6] = dA[IV3(p, i + 1, j, k, 1)];
+        dbf[7] = dA[IV3(p, i + 1, j, k, 2)];
+        dbf[8] = dB[IV3(p, i, j + 1, k, 0)];
+        dbf[9] = dB[IV3(p, i, j + 1, k, 2)];
+        dbf[10] = dC[IV3(p, i, j, k + 1, 0)];
+        dbf[11] = dC[IV3(p, i, j, k + 1, 1)];
+        El[0][0][0] = E[IV3(p, i, j, k, 0)];
+        El[0][0][1] = E[IV3(p, i, j, k + 1, 0)];
+        El[0][1][0] = E[IV3(p, i, j + 1, k, 0)];
+        El[0][1][1] = E[IV3(p, i, j + 1, k + 1, 0)];
+        El[1][0][0] = E[IV3(p, i, j, k, 1)];
+        El[1][0][1] = E[IV3(p, i, j, k + 1, 1)];
+        El[1][1][0] = E[IV3(p, i + 1, j, k, 1)];
+        El[1][1][1] = E[IV3(p, i + 1, j, k + 1, 1)];
+        El[2][0][0] = E[IV3(p, i, j, k, 2)];
+        El[2][0][1] = E[IV3(p, i, j + 1, k, 2)];
+        El[2][1][0] = E[IV3(p, i + 1, j, k, 2)];
+        El[2][1][1] = E[IV3(p, i + 1, j + 1, k, 2)];
+        eko_mhd3d_trace(p, q, dq, bf, dbf, (const double(*)[2][2])El, dtdx, dtdy, dtdz, qm, qp, qE);
+        for (int v = 0; v < NV; v++) {
+          const long o = IX3(p, i, j, k, v);
+          T[0][o] = qm[0][v];
+          T[1][o] = qm[1][v];
+          T[2][o] = qm[2][v];
+          T[3][o] = qp[0][v];
+          T[4][o] = qp[1][v];
+          T[5][o] = qp[2][v];
+          for (int e = 0; e < 4; e++)
+            for (int d = 0; d < 3; d++)
+              T[6 + 3 * e + d][o] = qE[e][d][v];
+        }
+      }
+  /* ComputeFluxesAndStoreFunctor3D_MHD (MHDRunFunctors3D.h:1783-1909) and
+   * ComputeEmfAndStoreFunctor3D (2100-2238) */
+#pragma omp parallel for collapse(2)
+  for (int i = gw; i < p->isize - gw + 1; i++)
+    for (int j = gw; j < p->jsize - gw + 1; j++)
+      for (int k = gw; k < p->ksize - gw + 1; k++) {
+        double ql[NV], qr[NV], f[NV], qe[4][NV];
+        for (int v = 0; v < NV; v++) {
+          ql[v] = T[0][IX3(p, i - 1, j, k, v)];
+          qr[v] = T[3][IX3(p, i, j, k, v)];
+        }
+        eko_riemann_mhd(p, ql, qr, f);
+        for (int v = 0; v < NV; v++)
+          F[0][IX3(p, i, j, k, v)] = f[v];
+        for (int v = 0; v < NV; v++) {
+          ql[v] = T[1][IX3(p, i, j - 1, k, v)];
+          qr[v] = T[4][IX3(p, i, j, k, v)];
+        }
+        swap2(ql, IU, IV);
+        swap2(ql, IA, IB);
+        swap2(qr, IU, IV);
+        swap2(qr, IA, IB);
+        eko_riemann_mhd(p, ql, qr, f);
+        for (int v = 0; v < NV; v++)
+          F[1][IX3(p, i, j, k, v)] = f[v];
+        for (int v = 0; v < NV; v++) {
+          ql[v] = T[2][IX3(p, i, j, k - 1, v)];
+          qr[v] = T[5][IX3(p, i, j, k, v)];
+        }
+        swap2(ql, IU, IW);
+        swap2(ql, IA, IC);
+        swap2(qr, IU, IW);
+        swap2(qr, IA, IC);
+        eko_riemann_mhd(p, ql, qr, f);
+        for (int v = 0; v < NV; v++)
+          F[2][IX3(p, i, j, k, v)] = f[v];
+        /* EMF z: RT3(i-1,j-1,k), RB3(i-1,j,k), LT3(i,j-1,k), LB3(i,j,k) */
+        for (int v = 0; v < NV; v++) {
+          qe[IRT][v] = T[6 + 3 * 0 + 2][IX3(p, i - 1, j - 1, k, v)];
+          qe[IRB][v] = T[6 + 3 * 1 + 2][IX3(p, i - 1, j, k, v)];
+          qe[ILT][v] = T[6 + 3 * 2 + 2][IX3(p, i, j - 1, k, v)];
+          qe[ILB][v] = T[6 + 3 * 3 + 2][IX3(p, i, j, k, v)];
+        }
+        M[IV3(p, i, j, k, 2)] = eko_compute_emf_dir(p, qe, 2);
+        /* EMF y: RT2(i-1,j,k-1), LT2(i,j,k-1) -> IRB, RB2(i-1,j,k) -> ILT, LB2(i,j,k) */
+        for (int v = 0; v < NV; v++) {
+          qe[IRT][v] = T[6 + 3 * 0 + 1][IX3(p, i - 1, j, k - 1, v)];
+          qe[IRB][v] = T[6 + 3 * 2 + 1][IX3(p, i, j, k - 1, v)];
+          qe[ILT][v] = T[6 + 3 * 1 + 1][IX3(p, i - 1, j, k, v)];
+          qe[ILB][v] = T[6 + 3 * 3 + 1][IX3(p, i, j, k, v)];
+        }
+        M[IV3(p, i, j, k, 1)] = eko_compute_emf_dir(p, qe, 1);
+        /* EMF x: RT(i,j-1,k-1), RB(i,j-1,k), LT(i,j,k-1), LB(i,j,k) */
+        for (int v = 0; v < NV; v++) {
+          qe[IRT][v] = T[6 + 3 * 0 + 0][IX3(p, i, j - 1, k - 1, v)];
+          qe[IRB][v] = T[6 + 3 * 1 + 0][IX3(p, i, j - 1, k, v)];
+          qe[ILT][v] = T[6 + 3 * 2 + 0][IX3(p, i, j, k - 1, v)];
+          qe[ILB][v] = T[6 + 3 * 3 + 0][IX3(p, i, j, k, v)];
+        }
+        M[IV3(p, i, j, k, 0)] = eko_compute_emf_dir(p, qe, 0);
+      }
+  /* UpdateFunctor3D_MHD + UpdateEmfFunctor3D (MHDRunFunctors3D.h:2430-2628) */
+#pragma omp parallel for collapse(2)
+  for (int i = gw; i < p->isize - gw; i++)
+    for (int j = gw; j < p->jsize - gw; j++)
+      for (int k = gw; k < p->ksize - gw; k++) {
+        double ud[NV];
+        for (int v = 0; v < NV; v++)
+          ud[v] = out[IX3(p, i, j, k, v)];
+        static const int hv[5] = { ID, IP, IU, IV, IW };
+        static const int my[5] = { ID, IP, IV, IU, IW };
+        static const int mz[5] = { ID, IP, IW, IV, IU };
+        for (int n = 0; n < 5; n++)
+          ud[hv[n]] += F[0][IX3(p, i, j, k, hv[n])] * dtdx;
+        for (int n = 0; n < 5; n++)
+          ud[hv[n]] -= F[0][IX3(p, i + 1, j, k, hv[n])] * dtdx;
+        for (int n = 0; n < 5; n++)
+          ud[hv[n]] += F[1][IX3(p, i, j, k, my[n])] * dtdy;
+        for (int n = 0; n < 5; n++)
+          ud[hv[n]] -= F[1][IX3(p, i, j + 1, k, my[n])] * dtdy;
+        for (int n = 0; n < 5; n++)
+          ud[hv[n]] += F[2][IX3(p, i, j, k, mz[n])] * dtdz;
+        for (int n = 0; n < 5; n++)
+          ud[hv[n]] -= F[2][IX3(p, i, j, k + 1, mz[n])] * dtdz;
+        ud[IA] += (M[IV3(p, i, j + 1, k, 2)] - M[IV3(p, i, j, k, 2)]) * dtdy;
+        ud[IB] -= (M[IV3(p, i + 1, j, k, 2)] - M[IV3(p, i, j, k, 2)]) * dtdx;
+        ud[IA] -= (M[IV3(p, i, j, k + 1, 1)] - M[IV3(p, i, j, k, 1)]) * dtdz;
+        ud[IB] += (M[IV3(p, i, j, k + 1, 0)] - M[IV3(p, i, j, k, 0)]) * dtdz;
+        ud[IC] += (M[IV3(p, i + 1, j, k, 1)] - M[IV3(p, i, j, k, 1)]) * dtdx;
+        ud[IC] -= (M[IV3(p, i, j + 1, k, 0)] - M[IV3(p, i, j, k, 0)]) * dtdy;
+        for (int v = 0; v < NV; v++)
+          out[IX3(p, i, j, k, v)] = ud[v];
+      }
+  free(Q);
+  free(E);
+  free(dA);
+  free(dB);
+  free(dC);
+  free(M);
+  for (int a = 0; a < 3; a++)
+    free(F[a]);
+  for (int a = 0; a < 18; a++)
+    free(T[a]);
+  return 0;
+}
+
+/* nsteps 3D MHD steps with a fixed dt */
+int eko_mhd3d_run_dt(const eko_params* p, double* U, int nsteps, double dt)
+{
+  const long n = (long)p->isize * p->jsize * p->ksize * NV;
+  double* U2 = (double*)malloc(sizeof(double) * n);
+  if (!U2)
+    return 1;
+  for (int s = 0; s < nsteps; s++) {
+    double* in = (s % 2 == 0) ? U : U2;
+    double* out = (s % 2 == 0) ? U2 : U;
+    eko_make_boundaries(p, in);
+    memcpy(out, in, sizeof(double) * n);
+    if (mhd3d_step(p, in, out, dt))
+      return 1;
+  }
+  if (nsteps % 2 == 1)
+    memcpy(U, U2, sizeof(double) * n);
+  free(U2);
+  return 0;
+}
+
+/* SolverMHDMuscl<3> time loop (dt from the primitives inside each step). */
+int eko_mhd3d_run(const eko_params* p, double* U, int nStepmax, double tEnd, double* t_io, int* it_io,
+                  double* dt_last)
+{
+  const long n = (long)p->isize * p->jsize * p->ksize * NV;
+  double* U2 = (double*)malloc(sizeof(double) * n);
+  double* Q = (double*)calloc(n, sizeof(double));
+  if (!U2 || !Q)
+    return 1;
+  double t = *t_io, dt = 0.0;
+  int it = *it_io;
+  const int it0 = it;
+  while (!(t >= (tEnd - 1e-14) || it >= nStepmax)) {
+    double* in = ((it - it0) % 2 == 0) ? U : U2;
+    double* out = ((it - it0) % 2 == 0) ? U2 : U;
+    eko_make_boundaries(p, in);
+    memcpy(out, in, sizeof(double) * n);
+    eko_mhd3d_convert_to_primitives(p, in, Q);
+    dt = p->cfl / eko_mhd3d_inv_dt(p, Q);
+    if (t + dt > tEnd)
+      dt = tEnd - t;
+    if (mhd3d_step(p, in, out, dt))
+      return 1;
+    ++it;
+    t += dt;
+  }
+  if ((it - it0) % 2 == 1)
+    memcpy(U, U2, sizeof(double) * n);
+  *t_io = t;
+  *it_io = it;
+  if (dt_last)
+    *dt_last = dt;
+  free(U2);
+  free(Q);
   return 0;
 }

@@ -87,6 +87,15 @@ def lib():
         L.eko_mhd_inv_dt.restype = C.c_double
         L.eko_mhd_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
         L.eko_mhd_run.restype = C.c_int
+        L.eko_mhd3d_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
+        L.eko_mhd3d_run.restype = C.c_int
+        L.eko_mhd_run_dt.argtypes = [P(EkoParams), D, C.c_int, C.c_double]
+        L.eko_mhd3d_run_dt.argtypes = [P(EkoParams), D, C.c_int, C.c_double]
+        L.eko_mhd3d_convert_to_primitives.argtypes = [P(EkoParams), D, D]
+        L.eko_mhd3d_inv_dt.argtypes = [P(EkoParams), D]
+        L.eko_mhd3d_inv_dt.restype = C.c_double
+        L.eko_compute_emf_dir.argtypes = [P(EkoParams), A8, C.c_int]
+        L.eko_compute_emf_dir.restype = C.c_double
         _L = L
     return _L
 
@@ -221,3 +230,33 @@ def mhd_trace_2d(p: EkoParams, qnb, bfnb, dtdx: float, dtdy: float):
     qe = np.zeros((4, 8))
     lib().eko_mhd_trace_2d(C.byref(p), dp(qnb), dp(bfnb), dtdx, dtdy, dp(qm), dp(qp), dp(qe))
     return qm, qp, qe
+
+
+# ---------------------------------------------------------------- 3D MHD
+
+
+def mhd3d_run(p: EkoParams, U: np.ndarray, nStepmax: int, tEnd: float, t: float = 0.0, it: int = 0):
+    """SolverMHDMuscl<3> time loop; U (reference layout) is advanced in place."""
+    tt, ii, dd = C.c_double(t), C.c_int(it), C.c_double()
+    rc = lib().eko_mhd3d_run(C.byref(p), dp(U), nStepmax, tEnd, C.byref(tt), C.byref(ii), C.byref(dd))
+    if rc:
+        raise MemoryError("oracle allocation failed")
+    return tt.value, ii.value, dd.value
+
+
+def mhd3d_inv_dt(p: EkoParams, U: np.ndarray) -> float:
+    Q = np.zeros_like(U)
+    lib().eko_mhd3d_convert_to_primitives(C.byref(p), dp(U), dp(Q))
+    return lib().eko_mhd3d_inv_dt(C.byref(p), dp(Q))
+
+
+def compute_emf_dir(p: EkoParams, qedge, direction: int) -> float:
+    q = np.ascontiguousarray(qedge, dtype=np.float64).reshape(4, 8)
+    return lib().eko_compute_emf_dir(C.byref(p), q.ctypes.data_as(C.POINTER(C.c_double * 8)), direction)
+
+
+def mhd_run_dt(p: EkoParams, U: np.ndarray, nsteps: int, dt: float) -> None:
+    """nsteps steps with a fixed dt (2D or 3D by p.dim), U advanced in place."""
+    f = lib().eko_mhd3d_run_dt if p.dim == 3 else lib().eko_mhd_run_dt
+    if f(C.byref(p), dp(U), nsteps, dt):
+        raise MemoryError("oracle allocation failed")

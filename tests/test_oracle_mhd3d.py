@@ -1,0 +1,156 @@
+"""Pins of the 3D MHD oracle (SolverMHDMuscl<3>, impl 0).
+
+The strongest pin is dimensional reduction: a 3D state that does not vary along one axis
+(and has no field or velocity along it) must evolve like the 2D MHD oracle -- itself
+pinned against exact and limiting solutions -- on the other two axes. Doing this for
+each of the three axes exercises every direction's trace, flux rotation and EMF mapping.
+Conservation, the discrete divergence of B and steadiness at rest complete the set.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from euler_kokkos_amd.params import make_params
+
+ID, IP, IU, IV, IW, IA, IB, IC = range(8)
+
+
+def P3(oracle, nx, ny, nz, **kw):
+    kw.setdefault("gamma0", "1.666")
+    kw.setdefault("riemann", "hlld")
+    kw.setdefault("slope_type", "2")
+    kw.setdefault("smallr", "1e-7")
+    kw.setdefault("smallc", "1e-7")
+    kw.setdefault("cfl", "0.8")
+    kw.setdefault("bc", (3,) * 6)
+    return oracle.from_ek(make_params("MHD_Muscl_3D", nx, ny, nz, **kw))
+
+
+def P2(oracle, nx, ny, **kw):
+    kw.setdefault("gamma0", "1.666")
+    kw.setdefault("riemann", "hlld")
+    kw.setdefault("slope_type", "2")
+    kw.setdefault("smallr", "1e-7")
+    kw.setdefault("smallc", "1e-7")
+    kw.setdefault("cfl", "0.8")
+    kw.setdefault("bc", (3,) * 6)
+    return oracle.from_ek(make_params("MHD_Muscl_2D", nx, ny, **kw))
+
+
+# axis maps: which 3D axes carry the 2D (x, y) axes, and the component permutation
+# (velocity and field components follow their axes; the uniform axis carries none)
+CASES = {
+    "z-uniform": ((0, 1), {IU: IU, IV: IV, IA: IA, IB: IB}),
+    "x-uniform": ((1, 2), {IU: IV, IV: IW, IA: IB, IB: IC}),
+    "y-uniform": ((2, 0), {IU: IW, IV: IU, IA: IC, IB: IA}),
+}
+
+
+def embed(U2, g2, g3, axes, comp, nthird):
+    """2D reference-layout state (ghosts g2) -> 3D reference-layout state (ghosts g3)."""
+    n1, n2 = U2.shape[0] - 2 * g2, U2.shape[1] - 2 * g2
+    I2 = U2[g2:-g2, g2:-g2]
+    shape = [0, 0, 0]
+    shape[axes[0]] = n1
+    shape[axes[1]] = n2
+    third = 3 - axes[0] - axes[1]
+    shape[third] = nthird
+    V = np.zeros([s + 2 * g3 for s in shape] + [8])
+    core = np.zeros(shape + [8])
+    src = I2
+    # place 2D (a, b) on (axes[0], axes[1])
+    perm = [None, None, None]
+    perm[axes[0]] = 0
+    perm[axes[1]] = 1
+    perm[third] = 2
+    src3 = np.broadcast_to(src[:, :, None, :], (n1, n2, nthird, 8))
+    src3 = np.transpose(src3, [perm[0], perm[1], perm[2], 3])
+    core[..., ID] = src3[..., ID]
+    core[..., IP] = src3[..., IP]
+    for c2, c3 in comp.items():
+        core[..., c3] = src3[..., c2]
+    V[g3:-g3, g3:-g3, g3:-g3] = core
+    return V
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_dimensional_reduction_to_2d(oracle, case):
+    axes, comp = CASES[case]
+    n1, n2, n3 = 20, 16, 4
+    p2 = P2(oracle, n1, n2)
+    U2 = oracle.orszag_tang(p2)
+    oracle.mhd_run(p2, U2, 3, 1e9)  # a generic 2D state (Bz = vz = 0)
+    sizes = [0, 0, 0]
+    sizes[axes[0]] = n1
+    sizes[axes[1]] = n2
+    sizes[3 - sum(axes)] = n3
+    p3 = P3(oracle, *sizes)
+    U3 = embed(U2, 3, 3, axes, comp, n3)
+    # the 3D CFL adds the z speed, so both runs take the same fixed dt
+    dt = 0.4 / oracle.mhd_inv_dt(p2, U2)
+    oracle.mhd_run_dt(p2, U2, 8, dt)
+    oracle.mhd_run_dt(p3, U3, 8, dt)
+    want = embed(U2, 3, 3, axes, comp, n3)[3:-3, 3:-3, 3:-3]
+    got = U3[3:-3, 3:-3, 3:-3]
+    scale = np.abs(want).max()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * scale)
+
+
+def blast3d(oracle, n, B0=0.5, bc=(3,) * 6):
+    p = P3(oracle, n, n, n, bc=bc)
+    U = np.zeros(oracle.ref_shape(p))
+    x = p.xmin + p.dx / 2 + (np.arange(p.isize) - 3) * p.dx
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    inside = (X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2 < 0.1 ** 2
+    b = B0 / np.sqrt(3.0)
+    U[..., ID] = 1.0
+    U[..., IA] = U[..., IB] = U[..., IC] = b
+    U[..., IP] = np.where(inside, 10.0, 0.1) / (p.gamma0 - 1) + 0.5 * 3 * b * b
+    return p, U
+
+
+def divb3(p, U):
+    g = 3
+    A, B, Cf = U[..., IA], U[..., IB], U[..., IC]
+    s = slice(g, -g)
+    d = ((A[g + 1:-g + 1, s, s] - A[s, s, s]) / p.dx + (B[s, g + 1:-g + 1, s] - B[s, s, s]) / p.dy
+         + (Cf[s, s, g + 1:-g + 1] - Cf[s, s, s]) / p.dz)
+    return np.abs(d).max()
+
+
+def test_blast3d_conservation_and_divb(oracle):
+    p, U = blast3d(oracle, 20)
+    oracle.lib().eko_make_boundaries(C.byref(p), oracle.dp(U))
+    s0 = U[3:-3, 3:-3, 3:-3].sum(axis=(0, 1, 2))
+    oracle.mhd3d_run(p, U, 6, 1e9)
+    oracle.lib().eko_make_boundaries(C.byref(p), oracle.dp(U))
+    s1 = U[3:-3, 3:-3, 3:-3].sum(axis=(0, 1, 2))
+    np.testing.assert_allclose(s1, s0, rtol=1e-12, atol=1e-11)
+    assert divb3(p, U) < 1e-12 / p.dx
+    assert np.isfinite(U).all()
+
+
+def test_at_rest_uniform_field_bitwise_steady_3d(oracle):
+    p = P3(oracle, 10, 8, 9, bc=(3, 3, 2, 2, 3, 3))
+    U = np.zeros(oracle.ref_shape(p))
+    U[..., ID] = 1.3
+    U[..., IA], U[..., IB], U[..., IC] = 0.2, -0.3, 0.4
+    U[..., IP] = 0.7 / (p.gamma0 - 1) + 0.5 * 0.29
+    ref = U[3:-3, 3:-3, 3:-3].copy()
+    oracle.mhd3d_run(p, U, 5, 1e9)
+    assert np.array_equal(U[3:-3, 3:-3, 3:-3], ref)
+
+
+def test_emf_dir_mapping_matches_permuted_emfz(oracle):
+    # compute_emf<EMFX>/<EMFY> are EMFZ on cyclically permuted components
+    p = P3(oracle, 8, 8, 8)
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        qe = np.array([[rng.uniform(0.5, 2), rng.uniform(0.3, 2), *rng.uniform(-0.5, 0.5, 3),
+                        *rng.uniform(-1, 1, 3)] for _ in range(4)])
+        for d, (cu, cv, ca, cb, cw, cc) in ((0, (IV, IW, IB, IC, IU, IA)), (1, (IW, IU, IC, IA, IV, IB))):
+            t = qe.copy()
+            t[:, IU], t[:, IV], t[:, IA], t[:, IB], t[:, IW], t[:, IC] = (qe[:, cu], qe[:, cv], qe[:, ca],
+                                                                        qe[:, cb], qe[:, cw], qe[:, cc])
+            assert oracle.compute_emf_dir(p, qe, d) == oracle.compute_emfz(p, t)
