@@ -49,6 +49,8 @@ UNITS = [
     ("ek_pool_fp64.hip", ["-ffp-contract=off"]),
     ("ek_pool_fp32.hip", FP32_FLAGS),
     ("ek_pool.hip", ["-ffp-contract=off"]),
+    ("ek_mhd3d_fp64.hip", ["-ffp-contract=off"]),
+    ("ek_mhd3d_fp32.hip", FP32_FLAGS),
 ]
 HOST_UNITS = ["ConfigMap.cpp", "HydroParams.cpp", "SolverBase.cpp", "SolverHydroMuscl.cpp", "SolverMHDMuscl.cpp",
               "SolverFactory.cpp", "Problems.cpp", "IO_VTK.cpp", "ek_host_capi.cpp"]

@@ -21,6 +21,7 @@ struct ek_ctx {
   unsigned long long* scratch_bits = nullptr;
   double* partial = nullptr;
   double* stage = nullptr;   // FP64 staging buffer for reference-layout transfers
+  void* work = nullptr;      // 3D MHD ghost-padded work area
   long long stage_elems = 0;
   int nblocks_budget = 0;
   size_t elem = 8;
@@ -84,9 +85,7 @@ int ek::validate_params(const ek_params* p, std::string& why)
   if (p->dim != 2 && p->dim != 3)
     return why = "dim must be 2 or 3", 1;
   if (p->mhd) {
-    // SolverMHDMuscl<2>, impl 0 (src/muscl/SolverMHDMuscl.cpp:369-454)
-    if (p->dim != 2)
-      return why = "MHD is provided in 2D (MHD_Muscl_2D) by this library build", 1;
+    // SolverMHDMuscl<2|3>, impl 0 (src/muscl/SolverMHDMuscl.cpp:369-512)
     if (p->gw != 3)
       return why = "MHD MUSCL needs ghostWidth 3", 1;
     if (p->nbvar != 8)
@@ -147,6 +146,14 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
   }
   c->bytes = (long long)(2 * state_bytes + 4 * sizeof(ek::TimeSlot) + 64 +
                          (size_t)c->nblocks_budget * 16 * sizeof(double));
+  if (c->p.mhd && c->p.dim == 3) {
+    const size_t wb = (size_t)ek::fp64::mhd3_scratch_elems(c->p.nx, c->p.ny, c->p.nz) * c->elem;
+    if (hipMalloc(&c->work, wb) != hipSuccess) {
+      ek_destroy(c);
+      return fail(nullptr, EK_ENOMEM, "device allocation of the 3D MHD work area failed");
+    }
+    c->bytes += (long long)wb;
+  }
   hipMemsetAsync(c->U[0], 0, state_bytes, c->stream);
   hipMemsetAsync(c->U[1], 0, state_bytes, c->stream);
   hipMemsetAsync(c->ts, 0, 4 * sizeof(ek::TimeSlot), c->stream);
@@ -177,6 +184,7 @@ extern "C" void ek_destroy(ek_ctx* c)
   hipFree(c->scratch_bits);
   hipFree(c->partial);
   hipFree(c->stage);
+  hipFree(c->work);
   if (c->stream)
     hipStreamDestroy(c->stream);
   delete c;
@@ -279,6 +287,11 @@ extern "C" int ek_download(ek_ctx* c, double* host, int layout)
 
 static hipError_t launch_inv_dt_into(ek_ctx* c, unsigned long long* dst)
 {
+  if (c->p.mhd && c->p.dim == 3) {
+    if (c->p.precision == EK_FP64)
+      return ek::fp64::mhd3_inv_dt((const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
+    return ek::fp32::mhd3_inv_dt((const float*)c->U[c->cur], c->g32, c->gas32, dst, c->stream);
+  }
   if (c->p.mhd) {
     if (c->p.precision == EK_FP64)
       return ek::fp64::mhd_inv_dt((const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
@@ -305,6 +318,13 @@ extern "C" int ek_inv_dt(ek_ctx* c, double* inv_dt)
 static hipError_t launch_one(ek_ctx* c, const ek::StepCtl& ctl, void* in, void* out)
 {
   const ek_params& p = c->p;
+  if (p.mhd && p.dim == 3) {
+    if (p.precision == EK_FP64)
+      return ek::fp64::mhd3_step(p.riemann, c->st, (const double*)in, (double*)out, (double*)c->work, c->g64,
+                                 c->gas64, ctl, c->stream);
+    return ek::fp32::mhd3_step(p.riemann, c->st, (const float*)in, (float*)out, (float*)c->work, c->g32, c->gas32,
+                               ctl, c->stream);
+  }
   if (p.mhd) {
     if (p.precision == EK_FP64)
       return ek::fp64::mhd_step(p.riemann, c->st, (const double*)in, (double*)out, c->g64, c->gas64, ctl,

@@ -34,6 +34,13 @@ hipError_t mhd_inv_dt(const double* U, const Geom<double>& geo, const Gas<double
 hipError_t pool_step(int dim, int variant, int st, const double* Ua, double* Ub, const PoolJob<double>* jobs,
                      const PoolWork* work, int nwork, TimeSlot* ts, int slot, int grid, hipStream_t s);
 int pool_occupancy(int dim, int variant, int st);
+// 3D MHD (ek_mhd3d_fp64.hip): a pipeline over a ghost-padded work area `scratch` of
+// mhd3_scratch_elems(nx, ny, nz) reals
+hipError_t mhd3_step(int rs, int st, const double* Uin, double* Uout, double* scratch, const Geom<double>& geo,
+                     const Gas<double>& gas, const StepCtl& ctl, hipStream_t s);
+hipError_t mhd3_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
+                       hipStream_t s);
+long long mhd3_scratch_elems(int nx, int ny, int nz);
 } // namespace fp64
 namespace fp32 {
 hipError_t step(int dim, int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo,
@@ -55,5 +62,12 @@ hipError_t mhd_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& 
 hipError_t pool_step(int dim, int variant, int st, const float* Ua, float* Ub, const PoolJob<float>* jobs,
                      const PoolWork* work, int nwork, TimeSlot* ts, int slot, int grid, hipStream_t s);
 int pool_occupancy(int dim, int variant, int st);
+// 3D MHD (ek_mhd3d_fp32.hip): a pipeline over a ghost-padded work area `scratch` of
+// mhd3_scratch_elems(nx, ny, nz) reals
+hipError_t mhd3_step(int rs, int st, const float* Uin, float* Uout, float* scratch, const Geom<float>& geo,
+                     const Gas<float>& gas, const StepCtl& ctl, hipStream_t s);
+hipError_t mhd3_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
+                       hipStream_t s);
+long long mhd3_scratch_elems(int nx, int ny, int nz);
 } // namespace fp32
 } // namespace ek

@@ -1,0 +1,550 @@
+// ek_mhd3d.h -- 3D MHD MUSCL-Hancock step (HLLD fluxes, 3D constrained transport) for gfx950.
+//
+// One step is SolverMHDMuscl<3>::godunov_unsplit_impl, impl 0 (src/muscl/SolverMHDMuscl.cpp:
+// 462-512), as a pipeline of kernels over a ghost-padded SoA work area (x fastest, 3 ghost
+// layers each side, so padded index (i,j,k) is the reference's U(i,j,k)):
+//   pad     : interior state -> padded state, ghosts through the boundary map
+//   prims   : primitives with the right/top/front face fields (ConvertToPrimitivesFunctor3D_MHD)
+//   efield  : edge electric fields and transverse face-field slopes (ComputeElecFieldFunctor3D,
+//             ComputeMagSlopesFunctor3D)
+//   trace   : the 6 face states and 12 edge states of every cell (trace_unsplit_mhd_3d_simpler);
+//             edge states keep the 7 components the edge solver reads, in its own order
+//   fluxemf : 3 Riemann problems and 3 edge EMFs per cell (ComputeFluxesAndStoreFunctor3D_MHD,
+//             ComputeEmfAndStoreFunctor3D)
+//   update  : flux update, CT update and the CFL term of the new state (UpdateFunctor3D_MHD,
+//             UpdateEmfFunctor3D, ComputeDtFunctor3D_MHD), with the device time bookkeeping
+// Every kernel reads the step's dt from the device time slot.
+#pragma once
+
+#include "ek_mhd.h"
+
+namespace ek {
+namespace mhd3 {
+
+using namespace ek::mhd;
+
+// Work-area layout. All arrays have PN = PX*PY*PZ elements.
+struct Work3 {
+  int PX, PY, PZ;
+  long long PN;
+  int nx, ny, nz;
+};
+
+// offsets (in arrays of PN elements) inside the scratch block
+enum {
+  A_UP = 0,           // padded state, 8
+  A_Q = 8,            // primitives, 8
+  A_E = 16,           // Ex, Ey, Ez
+  A_DB = 19,          // dA/dy, dA/dz, dB/dx, dB/dz, dC/dx, dC/dy (full, not halved)
+  A_FACE = 25,        // qm_x, qm_y, qm_z, qp_x, qp_y, qp_z: 6 x 8
+  A_EDGE = 73,        // (RT, RB, LT, LB) x (X, Y, Z): 12 x 7, order r p u v a b c of the edge solver
+  A_FLUX = 157,       // Fx, Fy, Fz: 3 x 5 (ID IP IU IV IW of the update, unrotated)
+  A_EMF = 172,        // EMF x, y, z
+  A_TOTAL = 175
+};
+
+__device__ __forceinline__ long long pidx(const Work3& w, int i, int j, int k)
+{
+  return ((long long)k * w.PY + j) * w.PX + i;
+}
+
+template <class T>
+__device__ __forceinline__ bool cell_of(const Work3& w, int& i, int& j, int& k)
+{
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= w.PN)
+    return false;
+  i = (int)(c % w.PX);
+  j = (int)((c / w.PX) % w.PY);
+  k = (int)(c / ((long long)w.PX * w.PY));
+  return true;
+}
+
+// step_begin without the finished-slot forwarding (the update kernel owns it)
+__device__ __forceinline__ bool step_dt(const StepCtl& c, double& dt)
+{
+  const TimeSlot cur = c.ts[c.slot];
+  if (c.mode == 0) {
+    if (cur.finished || cur.t >= (c.tEnd - 1e-14) || cur.it >= c.nStepmax)
+      return false;
+    dt = c.cfl / bits_to_double(cur.invdt_bits);
+    if (cur.t + dt > c.tEnd)
+      dt = c.tEnd - cur.t;
+  } else {
+    dt = c.dt_given;
+  }
+  return true;
+}
+
+// pad: ghost values as SolverBase::make_boundaries fills them (X, then Y, then Z passes;
+// reflecting axes negate the normal momentum and the normal face field)
+template <class T>
+__global__ __launch_bounds__(256) void pad_kernel(const T* __restrict__ U, T* __restrict__ S, Work3 w, Geom<T> geo,
+                                                  StepCtl ctl)
+{
+  double dt;
+  if (!step_dt(ctl, dt))
+    return;
+  int i, j, k;
+  if (!cell_of<T>(w, i, j, k))
+    return;
+  bool fx, fy, fz;
+  const int si = bc_map(i - 3, w.nx, geo.bc[0], geo.bc[1], fx);
+  const int sj = bc_map(j - 3, w.ny, geo.bc[2], geo.bc[3], fy);
+  const int sk = bc_map(k - 3, w.nz, geo.bc[4], geo.bc[5], fz);
+  const long long o = ((long long)sk * w.ny + sj) * w.nx + si;
+  const long long po = pidx(w, i, j, k);
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    T x = U[v * geo.vstride + o];
+    if ((fx && (v == IU || v == IA)) || (fy && (v == IV || v == IB)) || (fz && (v == IW || v == IC)))
+      x = x * T(-1);
+    S[(A_UP + v) * w.PN + po] = x;
+  }
+}
+
+// prims on i < PX-1, j < PY-1, k < PZ-1
+template <class T>
+__global__ __launch_bounds__(256) void prims_kernel(T* __restrict__ S, Work3 w, Gas<T> gas, StepCtl ctl)
+{
+  double dt;
+  if (!step_dt(ctl, dt))
+    return;
+  int i, j, k;
+  if (!cell_of<T>(w, i, j, k) || i >= w.PX - 1 || j >= w.PY - 1 || k >= w.PZ - 1)
+    return;
+  const long long po = pidx(w, i, j, k), PN = w.PN;
+  T u[NV], q[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++)
+    u[v] = S[(A_UP + v) * PN + po];
+  constoprim<T>(gas, u, S[(A_UP + IA) * PN + po + 1], S[(A_UP + IB) * PN + po + w.PX],
+                S[(A_UP + IC) * PN + po + (long long)w.PX * w.PY], q);
+#pragma unroll
+  for (int v = 0; v < NV; v++)
+    S[(A_Q + v) * PN + po] = q[v];
+}
+
+// edge electric fields and transverse face-field slopes on 1 <= i,j,k < size-1
+template <class T, int ST>
+__global__ __launch_bounds__(256) void efield_kernel(T* __restrict__ S, Work3 w, StepCtl ctl)
+{
+  double dt;
+  if (!step_dt(ctl, dt))
+    return;
+  int i, j, k;
+  if (!cell_of<T>(w, i, j, k) || i < 1 || j < 1 || k < 1 || i >= w.PX - 1 || j >= w.PY - 1 || k >= w.PZ - 1)
+    return;
+  const long long PN = w.PN, o = pidx(w, i, j, k);
+  const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
+  const T* Q = S + A_Q * PN;
+  const T* Up = S + A_UP * PN;
+  auto q = [&](int v, long long off) -> T { return Q[v * PN + o + off]; };
+  auto u = [&](int v, long long off) -> T { return Up[v * PN + o + off]; };
+  T vv, ww, uu, A, B, C;
+  vv = T(0.25) * (q(IV, -sy - sz) + q(IV, -sy) + q(IV, -sz) + q(IV, 0));
+  ww = T(0.25) * (q(IW, -sy - sz) + q(IW, -sy) + q(IW, -sz) + q(IW, 0));
+  B = T(0.5) * (u(IB, -sz) + u(IB, 0));
+  C = T(0.5) * (u(IC, -sy) + u(IC, 0));
+  S[(A_E + 0) * PN + o] = vv * C - ww * B;
+  uu = T(0.25) * (q(IU, -sx - sz) + q(IU, -sx) + q(IU, -sz) + q(IU, 0));
+  ww = T(0.25) * (q(IW, -sx - sz) + q(IW, -sx) + q(IW, -sz) + q(IW, 0));
+  A = T(0.5) * (u(IA, -sz) + u(IA, 0));
+  C = T(0.5) * (u(IC, -sx) + u(IC, 0));
+  S[(A_E + 1) * PN + o] = ww * A - uu * C;
+  uu = T(0.25) * (q(IU, -sx - sy) + q(IU, -sx) + q(IU, -sy) + q(IU, 0));
+  vv = T(0.25) * (q(IV, -sx - sy) + q(IV, -sx) + q(IV, -sy) + q(IV, 0));
+  A = T(0.5) * (u(IA, -sy) + u(IA, 0));
+  B = T(0.5) * (u(IB, -sx) + u(IB, 0));
+  S[(A_E + 2) * PN + o] = uu * B - vv * A;
+  // slope_unsplit_mhd_3d with xslope_type = min(slope_type, 2) (MHDBaseFunctor3D.h:563-668)
+  S[(A_DB + 0) * PN + o] = slope1<T, ST>(u(IA, 0), u(IA, sy), u(IA, -sy));
+  S[(A_DB + 1) * PN + o] = slope1<T, ST>(u(IA, 0), u(IA, sz), u(IA, -sz));
+  S[(A_DB + 2) * PN + o] = slope1<T, ST>(u(IB, 0), u(IB, sx), u(IB, -sx));
+  S[(A_DB + 3) * PN + o] = slope1<T, ST>(u(IB, 0), u(IB, sz), u(IB, -sz));
+  S[(A_DB + 4) * PN + o] = slope1<T, ST>(u(IC, 0), u(IC, sx), u(IC, -sx));
+  S[(A_DB + 5) * PN + o] = slope1<T, ST>(u(IC, 0), u(IC, sy), u(IC, -sy));
+}
+
+// the 3D trace (trace_unsplit_mhd_3d_simpler) on 1 <= i,j,k < size-2
+template <class T, int ST>
+__global__ __launch_bounds__(256) void trace_kernel(T* __restrict__ S, Work3 w, Geom<T> geo, Gas<T> gas, StepCtl ctl)
+{
+  double dt;
+  if (!step_dt(ctl, dt))
+    return;
+  int i, j, k;
+  if (!cell_of<T>(w, i, j, k) || i < 1 || j < 1 || k < 1 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
+    return;
+  const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
+  const long long PN = w.PN, o = pidx(w, i, j, k);
+  const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
+  const T* Q = S + A_Q * PN;
+  const T* Up = S + A_UP * PN;
+  const T* E = S + A_E * PN;
+  const T* DB = S + A_DB * PN;
+  T qc[NV], dqx[NV], dqy[NV], dqz[NV];
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    qc[v] = Q[v * PN + o];
+    dqx[v] = slope1<T, ST>(qc[v], Q[v * PN + o + sx], Q[v * PN + o - sx]);
+    dqy[v] = slope1<T, ST>(qc[v], Q[v * PN + o + sy], Q[v * PN + o - sy]);
+    dqz[v] = slope1<T, ST>(qc[v], Q[v * PN + o + sz], Q[v * PN + o - sz]);
+  }
+  const T ELL = E[o], ELR = E[o + sz], ERL = E[o + sy], ERR = E[o + sy + sz];
+  const T FLL = E[PN + o], FLR = E[PN + o + sz], FRL = E[PN + o + sx], FRR = E[PN + o + sx + sz];
+  const T GLL = E[2 * PN + o], GLR = E[2 * PN + o + sy], GRL = E[2 * PN + o + sx], GRR = E[2 * PN + o + sx + sy];
+  T r = qc[ID], p = qc[IP], u = qc[IU], v = qc[IV], ww = qc[IW], A = qc[IA], B = qc[IB], C = qc[IC];
+  T AL = Up[IA * PN + o], AR = Up[IA * PN + o + sx], BL = Up[IB * PN + o], BR = Up[IB * PN + o + sy];
+  T CL = Up[IC * PN + o], CR = Up[IC * PN + o + sz];
+  const T h = T(0.5);
+  const T drx = dqx[ID] * h, dpx = dqx[IP] * h, dux = dqx[IU] * h, dvx = dqx[IV] * h, dwx = dqx[IW] * h;
+  const T dCx = dqx[IC] * h, dBx = dqx[IB] * h;
+  const T dry = dqy[ID] * h, dpy = dqy[IP] * h, duy = dqy[IU] * h, dvy = dqy[IV] * h, dwy = dqy[IW] * h;
+  const T dCy = dqy[IC] * h, dAy = dqy[IA] * h;
+  const T drz = dqz[ID] * h, dpz = dqz[IP] * h, duz = dqz[IU] * h, dvz = dqz[IV] * h, dwz = dqz[IW] * h;
+  const T dAz = dqz[IA] * h, dBz = dqz[IB] * h;
+  const T dALy = h * DB[o], dALz = h * DB[PN + o], dBLx = h * DB[2 * PN + o], dBLz = h * DB[3 * PN + o];
+  const T dCLx = h * DB[4 * PN + o], dCLy = h * DB[5 * PN + o];
+  const T dARy = h * DB[o + sx], dARz = h * DB[PN + o + sx], dBRx = h * DB[2 * PN + o + sy];
+  const T dBRz = h * DB[3 * PN + o + sy], dCRx = h * DB[4 * PN + o + sz], dCRy = h * DB[5 * PN + o + sz];
+  const T dAx = h * (AR - AL), dBy = h * (BR - BL), dCz = h * (CR - CL);
+  const T gamma = gas.gamma0;
+  const T sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy + (-ww * drz - dwz * r) * dtdz;
+  const T su0 = (-u * dux - (dpx + B * dBx + C * dCx) / r) * dtdx + (-v * duy + B * dAy / r) * dtdy +
+                (-ww * duz + C * dAz / r) * dtdz;
+  const T sv0 = (-u * dvx + A * dBx / r) * dtdx + (-v * dvy - (dpy + A * dAy + C * dCy) / r) * dtdy +
+                (-ww * dvz + C * dBz / r) * dtdz;
+  const T sw0 = (-u * dwx + A * dCx / r) * dtdx + (-v * dwy + B * dCy / r) * dtdy +
+                (-ww * dwz - (dpz + A * dAz + B * dBz) / r) * dtdz;
+  const T sp0 = (-u * dpx - dux * gamma * p) * dtdx + (-v * dpy - dvy * gamma * p) * dtdy +
+                (-ww * dpz - dwz * gamma * p) * dtdz;
+  const T sA0 = (u * dBy + B * duy - v * dAy - A * dvy) * dtdy + (u * dCz + C * duz - ww * dAz - A * dwz) * dtdz;
+  const T sB0 = (v * dAx + A * dvx - u * dBx - B * dux) * dtdx + (v * dCz + C * dvz - ww * dBz - B * dwz) * dtdz;
+  const T sC0 = (ww * dAx + A * dwx - u * dCx - C * dux) * dtdx + (ww * dBy + B * dwy - v * dCy - C * dvy) * dtdy;
+  const T sAL0 = +(GLR - GLL) * dtdy * h - (FLR - FLL) * dtdz * h;
+  const T sAR0 = +(GRR - GRL) * dtdy * h - (FRR - FRL) * dtdz * h;
+  const T sBL0 = -(GRL - GLL) * dtdx * h + (ELR - ELL) * dtdz * h;
+  const T sBR0 = -(GRR - GLR) * dtdx * h + (ERR - ERL) * dtdz * h;
+  const T sCL0 = +(FRL - FLL) * dtdx * h - (ERL - ELL) * dtdy * h;
+  const T sCR0 = +(FRR - FLR) * dtdx * h - (ERR - ELR) * dtdy * h;
+  r = r + sr0;
+  u = u + su0;
+  v = v + sv0;
+  ww = ww + sw0;
+  p = p + sp0;
+  A = A + sA0;
+  B = B + sB0;
+  C = C + sC0;
+  AL = AL + sAL0;
+  AR = AR + sAR0;
+  BL = BL + sBL0;
+  BR = BR + sBR0;
+  CL = CL + sCL0;
+  CR = CR + sCR0;
+  const T smallR = gas.smallr, smallp = gas.smallp;
+  // face state f (0..5 = qm_x, qm_y, qm_z, qp_x, qp_y, qp_z); 3D floors p by smallp alone
+  auto face = [&](int f, T d0, T u0, T v0, T w0, T p0, T a0, T b0, T c0) {
+    T* F = S + (long long)(A_FACE + 8 * f) * PN + o;
+    d0 = emax(smallR, d0);
+    p0 = emax(smallp, p0);
+    F[ID * PN] = d0;
+    F[IU * PN] = u0;
+    F[IV * PN] = v0;
+    F[IW * PN] = w0;
+    F[IP * PN] = p0;
+    F[IA * PN] = a0;
+    F[IB * PN] = b0;
+    F[IC * PN] = c0;
+  };
+  face(3, r - drx, u - dux, v - dvx, ww - dwx, p - dpx, AL, B - dBx, C - dCx);
+  face(0, r + drx, u + dux, v + dvx, ww + dwx, p + dpx, AR, B + dBx, C + dCx);
+  face(4, r - dry, u - duy, v - dvy, ww - dwy, p - dpy, A - dAy, BL, C - dCy);
+  face(1, r + dry, u + duy, v + dvy, ww + dwy, p + dpy, A + dAy, BR, C + dCy);
+  face(5, r - drz, u - duz, v - dvz, ww - dwz, p - dpz, A - dAz, B - dBz, CL);
+  face(2, r + drz, u + duz, v + dvz, ww + dwz, p + dpz, A + dAz, B + dBz, CR);
+  // edge state e (0..3 = RT, RB, LT, LB) in direction d (0..2 = X, Y, Z), stored as the
+  // edge solver's (r, p, u, v, a, b, c): X -> (IV, IW, IB, IC, IA), Y -> (IW, IU, IC, IA, IB),
+  // Z -> (IU, IV, IA, IB, IC)
+  auto edge = [&](int e, int d, T d0, T u0, T v0, T w0, T p0, T a0, T b0, T c0) {
+    T* G = S + (long long)(A_EDGE + 7 * (3 * e + d)) * PN + o;
+    d0 = emax(smallR, d0);
+    p0 = emax(smallp, p0);
+    G[0] = d0;
+    G[PN] = p0;
+    if (d == 0) {
+      G[2 * PN] = v0; G[3 * PN] = w0; G[4 * PN] = b0; G[5 * PN] = c0; G[6 * PN] = a0;
+    } else if (d == 1) {
+      G[2 * PN] = w0; G[3 * PN] = u0; G[4 * PN] = c0; G[5 * PN] = a0; G[6 * PN] = b0;
+    } else {
+      G[2 * PN] = u0; G[3 * PN] = v0; G[4 * PN] = a0; G[5 * PN] = b0; G[6 * PN] = c0;
+    }
+  };
+  edge(0, 0, r + (+dry + drz), u + (+duy + duz), v + (+dvy + dvz), ww + (+dwy + dwz), p + (+dpy + dpz),
+       A + (+dAy + dAz), BR + (+dBRz), CR + (+dCRy));
+  edge(1, 0, r + (+dry - drz), u + (+duy - duz), v + (+dvy - dvz), ww + (+dwy - dwz), p + (+dpy - dpz),
+       A + (+dAy - dAz), BR + (-dBRz), CL + (+dCLy));
+  edge(2, 0, r + (-dry + drz), u + (-duy + duz), v + (-dvy + dvz), ww + (-dwy + dwz), p + (-dpy + dpz),
+       A + (-dAy + dAz), BL + (+dBLz), CR + (-dCRy));
+  edge(3, 0, r + (-dry - drz), u + (-duy - duz), v + (-dvy - dvz), ww + (-dwy - dwz), p + (-dpy - dpz),
+       A + (-dAy - dAz), BL + (-dBLz), CL + (-dCLy));
+  edge(0, 1, r + (+drx + drz), u + (+dux + duz), v + (+dvx + dvz), ww + (+dwx + dwz), p + (+dpx + dpz),
+       AR + (+dARz), B + (+dBx + dBz), CR + (+dCRx));
+  edge(1, 1, r + (+drx - drz), u + (+dux - duz), v + (+dvx - dvz), ww + (+dwx - dwz), p + (+dpx - dpz),
+       AR + (-dARz), B + (+dBx - dBz), CL + (+dCLx));
+  edge(2, 1, r + (-drx + drz), u + (-dux + duz), v + (-dvx + dvz), ww + (-dwx + dwz), p + (-dpx + dpz),
+       AL + (+dALz), B + (-dBx + dBz), CR + (-dCRx));
+  edge(3, 1, r + (-drx - drz), u + (-dux - duz), v + (-dvx - dvz), ww + (-dwx - dwz), p + (-dpx - dpz),
+       AL + (-dALz), B + (-dBx - dBz), CL + (-dCLx));
+  edge(0, 2, r + (+drx + dry), u + (+dux + duy), v + (+dvx + dvy), ww + (+dwx + dwy), p + (+dpx + dpy),
+       AR + (+dARy), BR + (+dBRx), C + (+dCx + dCy));
+  edge(1, 2, r + (+drx - dry), u + (+dux - duy), v + (+dvx - dvy), ww + (+dwx - dwy), p + (+dpx - dpy),
+       AR + (-dARy), BL + (+dBLx), C + (+dCx - dCy));
+  edge(2, 2, r + (-drx + dry), u + (-dux + duy), v + (-dvx + dvy), ww + (-dwx + dwy), p + (-dpx + dpy),
+       AL + (+dALy), BR + (-dBRx), C + (-dCx + dCy));
+  edge(3, 2, r + (-drx - dry), u + (-dux - duy), v + (-dvx - dvy), ww + (-dwx - dwy), p + (-dpx - dpy),
+       AL + (-dALy), BL + (-dBLx), C + (-dCx - dCy));
+}
+
+template <class T>
+__device__ __forceinline__ Edge<T> load_edge(const T* S, long long PN, int e, int d, long long o)
+{
+  const T* G = S + (long long)(A_EDGE + 7 * (3 * e + d)) * PN + o;
+  Edge<T> s;
+  s.r = G[0];
+  s.p = G[PN];
+  s.u = G[2 * PN];
+  s.v = G[3 * PN];
+  s.a = G[4 * PN];
+  s.b = G[5 * PN];
+  s.c = G[6 * PN];
+  return s;
+}
+
+// fluxes and EMFs on 3 <= i,j,k < size-2
+template <class T, int RS>
+__global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w, Gas<T> gas, StepCtl ctl)
+{
+  double dt;
+  if (!step_dt(ctl, dt))
+    return;
+  int i, j, k;
+  if (!cell_of<T>(w, i, j, k) || i < 3 || j < 3 || k < 3 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
+    return;
+  const long long PN = w.PN, o = pidx(w, i, j, k);
+  const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
+  T ql[NV], qr[NV], f[NV];
+  // x: qm_x(i-1) | qp_x(i)
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    ql[v] = S[(A_FACE + 0 + v) * PN + o - sx];
+    qr[v] = S[(A_FACE + 24 + v) * PN + o];
+  }
+  riemann<T, RS>(gas, ql, qr, f);
+  {
+    T* F = S + (A_FLUX + 0) * PN + o;
+    F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IU]; F[3 * PN] = f[IV]; F[4 * PN] = f[IW];
+  }
+  // y: qm_y(j-1) | qp_y(j), IU<->IV and IA<->IB around the solve
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    ql[v] = S[(A_FACE + 8 + v) * PN + o - sy];
+    qr[v] = S[(A_FACE + 32 + v) * PN + o];
+  }
+  {
+    T t;
+    t = ql[IU]; ql[IU] = ql[IV]; ql[IV] = t;
+    t = ql[IA]; ql[IA] = ql[IB]; ql[IB] = t;
+    t = qr[IU]; qr[IU] = qr[IV]; qr[IV] = t;
+    t = qr[IA]; qr[IA] = qr[IB]; qr[IB] = t;
+  }
+  riemann<T, RS>(gas, ql, qr, f);
+  {
+    T* F = S + (A_FLUX + 5) * PN + o;
+    F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IV]; F[3 * PN] = f[IU]; F[4 * PN] = f[IW];
+  }
+  // z: qm_z(k-1) | qp_z(k), IU<->IW and IA<->IC around the solve
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    ql[v] = S[(A_FACE + 16 + v) * PN + o - sz];
+    qr[v] = S[(A_FACE + 40 + v) * PN + o];
+  }
+  {
+    T t;
+    t = ql[IU]; ql[IU] = ql[IW]; ql[IW] = t;
+    t = ql[IA]; ql[IA] = ql[IC]; ql[IC] = t;
+    t = qr[IU]; qr[IU] = qr[IW]; qr[IW] = t;
+    t = qr[IA]; qr[IA] = qr[IC]; qr[IC] = t;
+  }
+  riemann<T, RS>(gas, ql, qr, f);
+  {
+    T* F = S + (A_FLUX + 10) * PN + o;
+    F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IW]; F[3 * PN] = f[IV]; F[4 * PN] = f[IU];
+  }
+  // EMF z: RT(i-1,j-1,k), RB(i-1,j,k), LT(i,j-1,k), LB(i,j,k) of the Z edges
+  S[(A_EMF + 2) * PN + o] = emf_z<T>(gas, load_edge(S, PN, 0, 2, o - sx - sy), load_edge(S, PN, 1, 2, o - sx),
+                                     load_edge(S, PN, 2, 2, o - sy), load_edge(S, PN, 3, 2, o));
+  // EMF y: RT(i-1,j,k-1), LT(i,j,k-1) in the RB place, RB(i-1,j,k) in the LT place, LB(i,j,k)
+  S[(A_EMF + 1) * PN + o] = emf_z<T>(gas, load_edge(S, PN, 0, 1, o - sx - sz), load_edge(S, PN, 2, 1, o - sz),
+                                     load_edge(S, PN, 1, 1, o - sx), load_edge(S, PN, 3, 1, o));
+  // EMF x: RT(i,j-1,k-1), RB(i,j-1,k), LT(i,j,k-1), LB(i,j,k)
+  S[(A_EMF + 0) * PN + o] = emf_z<T>(gas, load_edge(S, PN, 0, 0, o - sy - sz), load_edge(S, PN, 1, 0, o - sy),
+                                     load_edge(S, PN, 2, 0, o - sz), load_edge(S, PN, 3, 0, o));
+}
+
+// CFL term of a 3D cell from its new state and new right/top/front face fields
+template <class T>
+__device__ __forceinline__ double cfl3(const Gas<T>& gas, const T* u, T mx, T my, T mz, const Geom<T>& geo)
+{
+  if constexpr (sizeof(T) == 8) {
+    T q[NV];
+    constoprim<T>(gas, u, mx, my, mz, q);
+    const T d = q[ID], p = q[IP], a = q[IA], b = q[IB], c = q[IC];
+    const T b2 = a * a + b * b + c * c;
+    const T c2 = gas.gamma0 * p / d;
+    const T d2 = T(0.5) * (b2 / d + c2);
+    const T vx = esqrt(d2 + esqrt(d2 * d2 - c2 * a * a / d)) + eabs(q[IU]);
+    const T vy = esqrt(d2 + esqrt(d2 * d2 - c2 * b * b / d)) + eabs(q[IV]);
+    const T vz = esqrt(d2 + esqrt(d2 * d2 - c2 * c * c / d)) + eabs(q[IW]);
+    if (geo.pow2)
+      return vx * geo.idx + vy * geo.idy + vz * geo.idz;
+    return vx / geo.dx + vy / geo.dy + vz / geo.dz;
+  } else {
+    Gas<double> g;
+    g.gamma0 = gas.gamma0;
+    g.gm1 = gas.gm1;
+    g.smallr = gas.smallr;
+    g.smallp = gas.smallp;
+    double ud[NV], q[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      ud[v] = (double)u[v];
+    constoprim<double>(g, ud, (double)mx, (double)my, (double)mz, q);
+    const double d = q[ID], p = q[IP], a = q[IA], b = q[IB], c = q[IC];
+    const double b2 = a * a + b * b + c * c;
+    const double c2 = g.gamma0 * p / d;
+    const double d2 = 0.5 * (b2 / d + c2);
+    const double vx = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d)) + fabs(q[IU]);
+    const double vy = sqrt(d2 + sqrt(d2 * d2 - c2 * b * b / d)) + fabs(q[IV]);
+    const double vz = sqrt(d2 + sqrt(d2 * d2 - c2 * c * c / d)) + fabs(q[IW]);
+    return vx / geo.dx + vy / geo.dy + vz / geo.dz;
+  }
+}
+
+// update on the interior (UpdateFunctor3D_MHD + UpdateEmfFunctor3D) into the SoA output
+template <class T>
+__global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T* __restrict__ Uout, Work3 w,
+                                                     Geom<T> geo, Gas<T> gas, StepCtl ctl)
+{
+  __shared__ double sred[4];
+  double dt;
+  TimeSlot cur;
+  const bool first_block = blockIdx.x == 0;
+  if (!step_begin(ctl, first_block, dt, cur))
+    return;
+  const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
+  const long long n = geo.vstride;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double local = 0.0;
+  if (c < n) {
+    const int ii = (int)(c % w.nx), jj = (int)((c / w.nx) % w.ny), kk = (int)(c / ((long long)w.nx * w.ny));
+    const long long PN = w.PN, o = pidx(w, ii + 3, jj + 3, kk + 3);
+    const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
+    T u[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      u[v] = S[(A_UP + v) * PN + o];
+    constexpr int hv[5] = { ID, IP, IU, IV, IW };
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      T a = u[hv[m]];
+      a += S[(A_FLUX + m) * PN + o] * dtdx;
+      a -= S[(A_FLUX + m) * PN + o + sx] * dtdx;
+      a += S[(A_FLUX + 5 + m) * PN + o] * dtdy;
+      a -= S[(A_FLUX + 5 + m) * PN + o + sy] * dtdy;
+      a += S[(A_FLUX + 10 + m) * PN + o] * dtdz;
+      a -= S[(A_FLUX + 10 + m) * PN + o + sz] * dtdz;
+      u[hv[m]] = a;
+    }
+    const T* EX = S + (A_EMF + 0) * PN;
+    const T* EY = S + (A_EMF + 1) * PN;
+    const T* EZ = S + (A_EMF + 2) * PN;
+    u[IA] += (EZ[o + sy] - EZ[o]) * dtdy;
+    u[IB] -= (EZ[o + sx] - EZ[o]) * dtdx;
+    u[IA] -= (EY[o + sz] - EY[o]) * dtdz;
+    u[IB] += (EX[o + sz] - EX[o]) * dtdz;
+    u[IC] += (EY[o + sx] - EY[o]) * dtdx;
+    u[IC] -= (EX[o + sy] - EX[o]) * dtdy;
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      Uout[v * n + c] = u[v];
+    // new face fields of the right / top / front neighbours, as the next ghost fill sees them
+    T ar, bt, cf;
+    if (ii + 1 < w.nx || geo.bc[1] == BC_PERIODIC) {
+      const long long q = o + sx;
+      ar = S[(A_UP + IA) * PN + q];
+      ar += (EZ[q + sy] - EZ[q]) * dtdy;
+      ar -= (EY[q + sz] - EY[q]) * dtdz;
+    } else {
+      ar = geo.bc[1] == BC_DIRICHLET ? u[IA] * T(-1) : u[IA];
+    }
+    if (jj + 1 < w.ny || geo.bc[3] == BC_PERIODIC) {
+      const long long q = o + sy;
+      bt = S[(A_UP + IB) * PN + q];
+      bt -= (EZ[q + sx] - EZ[q]) * dtdx;
+      bt += (EX[q + sz] - EX[q]) * dtdz;
+    } else {
+      bt = geo.bc[3] == BC_DIRICHLET ? u[IB] * T(-1) : u[IB];
+    }
+    if (kk + 1 < w.nz || geo.bc[5] == BC_PERIODIC) {
+      const long long q = o + sz;
+      cf = S[(A_UP + IC) * PN + q];
+      cf += (EY[q + sx] - EY[q]) * dtdx;
+      cf -= (EX[q + sy] - EX[q]) * dtdy;
+    } else {
+      cf = geo.bc[5] == BC_DIRICHLET ? u[IC] * T(-1) : u[IC];
+    }
+    local = cfl3<T>(gas, u, ar, bt, cf, geo);
+  }
+  block_max_to<256>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
+  if (first_block && threadIdx.x == 0)
+    step_end_bookkeeping(ctl, cur, dt);
+}
+
+// CFL reduction over a 3D MHD state (interior SoA), neighbours' face fields through the
+// boundary map
+template <class T>
+__global__ __launch_bounds__(256) void inv_dt3_kernel(const T* __restrict__ U, Geom<T> geo, Gas<T> gas,
+                                                      unsigned long long* dst)
+{
+  __shared__ double sred[4];
+  const long long n = geo.vstride;
+  const int nx = geo.nx, ny = geo.ny, nz = geo.nz;
+  double local = 0.0;
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long long)nx * ny));
+    T u[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+      u[v] = U[v * n + c];
+    bool fx, fy, fz;
+    const int si = bc_map(i + 1, nx, geo.bc[0], geo.bc[1], fx);
+    const int sj = bc_map(j + 1, ny, geo.bc[2], geo.bc[3], fy);
+    const int sk = bc_map(k + 1, nz, geo.bc[4], geo.bc[5], fz);
+    T mx = U[IA * n + ((long long)k * ny + j) * nx + si];
+    T my = U[IB * n + ((long long)k * ny + sj) * nx + i];
+    T mz = U[IC * n + ((long long)sk * ny + j) * nx + i];
+    if (fx)
+      mx = mx * T(-1);
+    if (fy)
+      my = my * T(-1);
+    if (fz)
+      mz = mz * T(-1);
+    const double s = cfl3<T>(gas, u, mx, my, mz, geo);
+    local = (s > local || !(s == s)) ? s : local;
+  }
+  block_max_to<256>(dst, local, sred);
+}
+
+} // namespace mhd3
+} // namespace ek

@@ -106,6 +106,26 @@ def orszag_tang(p) -> np.ndarray:
     return np.ascontiguousarray(U[:, g:-g, g:-g])
 
 
+def mhd_blast3d(p, B0=1.0, radius="0.1", center=("0.5", "0.5", "0.5"), density="1.0",
+                pressure_in="10.0", pressure_out="0.1") -> np.ndarray:
+    """3D MHD blast with a uniform field B = B0 (1,1,1)/sqrt(3) on every face: rho = 1, v = 0,
+    p = pressure_in inside the sphere, pressure_out outside; E = p/(gamma-1) + B^2/2.
+    (The reference's InitBlastFunctor3D_MHD reads its field from BlastParams, which leaves
+    bz undefined, src/shared/problems/BlastParams.h:47-52; this builder states B fully.)"""
+    r = getfloat(radius)
+    cx, cy, cz = (getfloat(c) for c in center)
+    x, y, z = _grid(p)
+    d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz)
+    inside = d2 < r * r
+    b = float(B0) / np.sqrt(3.0)
+    U = np.zeros((8,) + x.shape)
+    U[0] = getfloat(density)
+    U[5] = U[6] = U[7] = b
+    pin, pout = getfloat(pressure_in), getfloat(pressure_out)
+    U[1] = np.where(inside, pin, pout) / (p.gamma0 - 1.0) + 0.5 * (b * b + b * b + b * b)
+    return U
+
+
 def soa_from_reference(p, R: np.ndarray) -> np.ndarray:
     """Interior of a reference-layout array U(i,j[,k],var) as the device SoA U[var][k][j][i]."""
     g = p.gw
