@@ -228,9 +228,11 @@ def serving_c5(n_jobs, rate, seed, steps):
     one job at a time."""
     from euler_kokkos_amd import native, serving
     res = {"workload": f"{n_jobs} 3D jobs, Poisson arrivals (mean {rate:g} jobs/s, seed {seed}), edges "
-                       f"uniform in [64,192], {steps} steps each; kinds: hydro Kelvin-Helmholtz shear layer "
-                       f"(HLLC + MC, gamma 1.4, CFL 0.5, periodic)",
-           "kinds": ["hydro_kh"]}
+                       f"uniform in [64,192], {steps} steps each; half 3D MHD blasts (rho 1, p 0.1 / 10 in r < 0.1, "
+                       f"uniform B = B0 (1,1,1)/sqrt(3), B0 uniform in [0.1,1], gamma 5/3, HLLD + CT, CFL 0.8), "
+                       f"half hydro Kelvin-Helmholtz shear layers (HLLC + MC, gamma 1.4, CFL 0.5); periodic; "
+                       f"FP32 state, FP64 dt",
+           "kinds": ["mhd_blast", "hydro_kh"]}
     jobs = serving.poisson_stream(n_jobs, rate, seed, steps=steps, precision=native.EK_FP32)
     for name, burst in (("stream", False), ("burst", True)):
         # stream: the Poisson arrival times (latency at the offered load); burst: every job

@@ -155,10 +155,55 @@ void init_orszag_tang(const HydroParams& p, const ConfigMap& c, std::vector<doub
         U[s.at(v, i, j, 0)] = at(i + gw, j + gw, v);
 }
 
+// InitBlastFunctor3D_MHD (src/muscl/MHDInitFunctors3D.h:267-371) with BlastParams
+// (src/shared/problems/BlastParams.h:28-53). BlastParams reads the by keys and then
+// overwrites by_in / by_out with the bz_in / bz_out keys, leaving bz_in / bz_out unset; this
+// build keeps that key mapping (By takes the bz keys) and sets Bz to 0.
+void init_blast_mhd3d(const HydroParams& p, const ConfigMap& c, std::vector<double>& U)
+{
+  const double xmin = c.getFloat("mesh", "xmin", 0.0), ymin = c.getFloat("mesh", "ymin", 0.0);
+  const double zmin = c.getFloat("mesh", "zmin", 0.0), xmax = c.getFloat("mesh", "xmax", 1.0);
+  const double ymax = c.getFloat("mesh", "ymax", 1.0), zmax = c.getFloat("mesh", "zmax", 1.0);
+  const double radius = c.getFloat("blast", "radius", (xmin + xmax) / 2.0 / 10);
+  const double cx = c.getFloat("blast", "center_x", (xmin + xmax) / 2);
+  const double cy = c.getFloat("blast", "center_y", (ymin + ymax) / 2);
+  const double cz = c.getFloat("blast", "center_z", (zmin + zmax) / 2);
+  const double din = c.getFloat("blast", "density_in", 1.0);
+  const double dout = c.getFloat("blast", "density_out", 1.2);
+  const double pin = c.getFloat("blast", "pressure_in", 10.0);
+  const double pout = c.getFloat("blast", "pressure_out", 0.1);
+  const double bxi = c.getFloat("blast", "bx_in", 0.5), byi = c.getFloat("blast", "bz_in", 0.5);
+  const double bxo = c.getFloat("blast", "bx_out", 0.5), byo = c.getFloat("blast", "bz_out", 0.5);
+  const double bzi = 0.0, bzo = 0.0;
+  const Shape s = shape_of(p);
+  U.assign((size_t)(s.cells * 8), 0.0);
+  const double gamma0 = p.settings.gamma0;
+  const double radius2 = radius * radius;
+  for (int k = 0; k < s.nz; k++)
+    for (int j = 0; j < s.ny; j++)
+      for (int i = 0; i < s.nx; i++) {
+        const double x = p.xmin + p.dx / 2 + i * p.dx;
+        const double y = p.ymin + p.dy / 2 + j * p.dy;
+        const double z = p.zmin + p.dz / 2 + k * p.dz;
+        const double d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+        const bool in = d2 < radius2;
+        const double a = in ? bxi : bxo, b = in ? byi : byo, cc = in ? bzi : bzo;
+        U[s.at(ID, i, j, k)] = in ? din : dout;
+        U[s.at(IA, i, j, k)] = a;
+        U[s.at(IB, i, j, k)] = b;
+        U[s.at(IC, i, j, k)] = cc;
+        U[s.at(IP, i, j, k)] = (in ? pin : pout) / (gamma0 - 1.0) + 0.5 * (a * a + b * b + cc * cc);
+      }
+}
+
 bool init_problem_mhd(const std::string& problem, const HydroParams& params, const ConfigMap& configMap,
                       std::vector<double>& U)
 {
-  if (problem == "orszag_tang") {
+  if (problem == "blast" && params.dimType == THREE_D) {
+    init_blast_mhd3d(params, configMap, U);
+    return true;
+  }
+  if (problem == "orszag_tang" && params.dimType == TWO_D) {
     init_orszag_tang(params, configMap, U);
     return true;
   }

@@ -9,6 +9,7 @@
 //   blast:   BlastParams (src/shared/problems/BlastParams.h:28-53),
 //            InitBlastFunctor2D/3D (src/muscl/HydroInitFunctors2D.h:127-209, 3D.h:173-265)
 //   orszag_tang (MHD 2D): InitOrszagTangFunctor2D (src/muscl/MHDInitFunctors2D.h:345-479)
+//   blast (MHD 3D): InitBlastFunctor3D_MHD (src/muscl/MHDInitFunctors3D.h:267-371)
 #pragma once
 
 #include <string>
@@ -30,5 +31,6 @@ void init_blast(const HydroParams& params, const ConfigMap& configMap, std::vect
 bool init_problem_mhd(const std::string& problem, const HydroParams& params, const ConfigMap& configMap,
                       std::vector<double>& U);
 void init_orszag_tang(const HydroParams& params, const ConfigMap& configMap, std::vector<double>& U);
+void init_blast_mhd3d(const HydroParams& params, const ConfigMap& configMap, std::vector<double>& U);
 
 } // namespace euler_kokkos

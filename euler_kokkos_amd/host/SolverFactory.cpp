@@ -14,6 +14,7 @@ SolverFactory::SolverFactory()
   registerSolver("Hydro_Muscl_2D", &muscl::SolverHydroMuscl<2>::create);
   registerSolver("Hydro_Muscl_3D", &muscl::SolverHydroMuscl<3>::create);
   registerSolver("MHD_Muscl_2D", &muscl::SolverMHDMuscl<2>::create);
+  registerSolver("MHD_Muscl_3D", &muscl::SolverMHDMuscl<3>::create);
 }
 
 SolverFactory& SolverFactory::Instance()

@@ -32,9 +32,15 @@ SolverMHDMuscl<dim>::SolverMHDMuscl(HydroParams& params, ConfigMap& configMap)
   std::vector<double> U;
   if (!init_problem_mhd(m_problem_name, params, configMap, U)) {
     std::cout << "Problem : " << m_problem_name << " is not recognized / implemented." << std::endl;
-    std::cout << "Use default - Orszag-Tang vortex" << std::endl;
-    m_problem_name = "orszag_tang";
-    init_orszag_tang(params, configMap, U);
+    if (dim == 2) {
+      std::cout << "Use default - Orszag-Tang vortex" << std::endl;
+      m_problem_name = "orszag_tang";
+      init_orszag_tang(params, configMap, U);
+    } else {
+      std::cout << "Use default - blast" << std::endl;
+      m_problem_name = "blast";
+      init_blast_mhd3d(params, configMap, U);
+    }
   }
   check(ek_upload(m_ctx, U.data(), EK_LAYOUT_SOA), "ek_upload");
   compute_dt();
@@ -137,6 +143,7 @@ void SolverMHDMuscl<dim>::save_solution_impl()
 }
 
 template class SolverMHDMuscl<2>;
+template class SolverMHDMuscl<3>;
 
 } // namespace muscl
 } // namespace euler_kokkos

@@ -1,8 +1,8 @@
-// SolverMHDMuscl.h -- 2D MHD MUSCL-Hancock solver (HLLD + constrained transport) backed by
-// the HIP library.
+// SolverMHDMuscl.h -- MHD MUSCL-Hancock solver (2D/3D, HLLD + constrained transport) backed
+// by the HIP library.
 //
-// Drop-in for euler_kokkos' muscl::SolverMHDMuscl<2> (src/muscl/SolverMHDMuscl.h:44-903,
-// SolverMHDMuscl.cpp:369-454): registered as "MHD_Muscl_2D", same .ini keys, time loop and
+// Drop-in for euler_kokkos' muscl::SolverMHDMuscl<dim> (src/muscl/SolverMHDMuscl.h:44-903,
+// SolverMHDMuscl.cpp:369-512): registered as "MHD_Muscl_2D" / "MHD_Muscl_3D", same .ini keys, time loop and
 // output. As in the reference the time step of an iteration is computed inside
 // godunov_unsplit, from the primitives of the state it advances.
 #pragma once
@@ -33,7 +33,7 @@ public:
 
   void godunov_unsplit();
 
-  // current state, SoA interior [var][j][i] (FP64)
+  // current state, SoA interior [var][k][j][i] (FP64)
   std::vector<double> state() const;
   ek_ctx* context() { return m_ctx; }
 

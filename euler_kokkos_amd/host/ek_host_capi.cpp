@@ -91,6 +91,8 @@ extern "C" int ek_ini_run(const char* path, double* state, long long n, double* 
       U = h3->state();
     else if (auto* m2 = dynamic_cast<muscl::SolverMHDMuscl<2>*>(s))
       U = m2->state();
+    else if (auto* m3 = dynamic_cast<muscl::SolverMHDMuscl<3>*>(s))
+      U = m3->state();
     if ((long long)U.size() != n) {
       delete s;
       g_err = "state buffer size mismatch";

@@ -27,13 +27,14 @@ from .pool import MT19937_64, Pool, PoolFull
 @dataclass
 class Job:
     id: int
-    kind: str          # "hydro_kh" (3D hydro Kelvin-Helmholtz shear layer)
+    kind: str          # "hydro_kh" (3D hydro Kelvin-Helmholtz shear layer) or "mhd_blast"
     n: int             # grid edge (n^3 cells)
     steps: int
     arrival: float     # seconds after the stream starts
     seed: int
     params: object = None
     U0: np.ndarray = None
+    B0: float = 0.0    # field strength of an MHD blast
     admit_t: float = -1.0
     done_t: float = -1.0
     result: np.ndarray = field(default=None, repr=False)
@@ -68,10 +69,19 @@ def job_params(kind: str, n: int, precision: int):
     if kind == "hydro_kh":
         return make_params("Hydro_Muscl_3D", n, n, n, gamma0="1.4", cfl="0.5", slope_type="2",
                            riemann="hllc", bc=(3,) * 6, precision=precision)
+    if kind == "mhd_blast":
+        return make_params("MHD_Muscl_3D", n, n, n, gamma0="1.6666667", cfl="0.8", slope_type="2",
+                           riemann="hlld", bc=(3,) * 6, precision=precision)
     raise ValueError(f"unknown job kind {kind}")
 
 
-def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh",), edge=(64, 192), steps=50,
+def job_state(j: "Job") -> np.ndarray:
+    if j.kind == "hydro_kh":
+        return kh3d(j.params, j.seed)
+    return problems.mhd_blast3d(j.params, B0=j.B0)
+
+
+def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh", "mhd_blast"), edge=(64, 192), steps=50,
                    precision: int = native.EK_FP32, build_state: bool = True):
     """Seeded job stream: exponential inter-arrival times of mean 1/rate seconds, kind
     uniform over `kinds`, edge uniform over the integers [edge[0], edge[1]]."""
@@ -84,21 +94,79 @@ def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh",), edg
         n = edge[0] + min(edge[1] - edge[0], int(rng.uniform() * (edge[1] - edge[0] + 1)))
         js = rng.next_u64() & 0x7FFFFFFF
         j = Job(i, kind, n, steps, t, js)
+        j.B0 = 0.1 + 0.9 * rng.uniform()
         j.params = job_params(kind, n, precision)
         if build_state:
-            j.U0 = kh3d(j.params, js)
+            j.U0 = job_state(j)
         jobs.append(j)
     return jobs
 
 
-class Server:
-    """Pools per job kind sharing an HBM budget; continuous batching over arrivals."""
+class ContextGroup:
+    """Jobs that each run in their own device context (3D MHD: the step pipeline needs a
+    per-job ghost-padded work area), admitted against a byte budget. Same interface as a
+    Pool: admit / step / status / download / retire."""
 
-    def __init__(self, hbm_bytes: int, precision: int = native.EK_FP32, kinds=("hydro_kh",), max_jobs=512):
+    WORK_ARRAYS = 175  # reals per padded cell in the 3D MHD work area
+
+    def __init__(self, budget_bytes: int):
+        self.budget = int(budget_bytes)
+        self.used = 0
+        self.jobs = {}
+        self.next_id = 0
+
+    def job_bytes(self, p) -> int:
+        elem = 8 if p.precision == native.EK_FP64 else 4
+        cells = p.nx * p.ny * p.nz
+        padded = (p.nx + 6) * (p.ny + 6) * (p.nz + 6)
+        return elem * (2 * 8 * cells + self.WORK_ARRAYS * padded) + (1 << 20)
+
+    def admit(self, p, U, nStepmax, tEnd=1e30):
+        need = self.job_bytes(p)
+        if self.used + need > self.budget:
+            raise PoolFull("context group budget exhausted")
+        ctx = native.Context(p)
+        ctx.upload(U, native.EK_LAYOUT_SOA)
+        jid = self.next_id
+        self.next_id += 1
+        self.jobs[jid] = dict(ctx=ctx, t=0.0, it=0, nStepmax=nStepmax, tEnd=tEnd, need=need)
+        self.used += need
+        return jid
+
+    def step(self, nsteps=1):
+        for j in self.jobs.values():
+            if j["it"] < j["nStepmax"] and j["t"] < j["tEnd"] - 1e-14:
+                j["t"], j["it"], _ = j["ctx"].run(nsteps, j["nStepmax"], j["tEnd"], j["t"], j["it"])
+
+    def status(self, jid):
+        j = self.jobs[jid]
+        return j["t"], j["it"], j["it"] >= j["nStepmax"] or j["t"] >= j["tEnd"] - 1e-14
+
+    def download(self, jid):
+        return self.jobs[jid]["ctx"].download(native.EK_LAYOUT_SOA)
+
+    def retire(self, jid):
+        j = self.jobs.pop(jid)
+        j["ctx"].close()
+        self.used -= j["need"]
+
+    def close(self):
+        for jid in list(self.jobs):
+            self.retire(jid)
+
+
+class Server:
+    """Job pools per kind sharing an HBM budget; continuous batching over arrivals."""
+
+    def __init__(self, hbm_bytes: int, precision: int = native.EK_FP32, kinds=("hydro_kh", "mhd_blast"),
+                 max_jobs=512):
         self.pools = {}
         share = hbm_bytes // len(kinds)
         for k in kinds:
-            self.pools[k] = Pool(job_params(k, 64, precision), share, max_jobs)
+            if k == "mhd_blast":
+                self.pools[k] = ContextGroup(share)
+            else:
+                self.pools[k] = Pool(job_params(k, 64, precision), share, max_jobs)
         self.precision = precision
         self.step_s = 0.0  # wall time of one pool step (all busy pools), last measured
 

@@ -133,3 +133,58 @@ def test_ini_orszag_tang_mhd2d_matches_oracle(tmp_path, oracle):
     assert it == it_o == 25 and t == t_o
     want = U[3:-3, 3:-3, :].transpose(2, 1, 0).reshape(-1)
     np.testing.assert_array_equal(G, want)
+
+
+INI_MHD3D = """[run]
+solver_name=MHD_Muscl_3D
+tEnd=1e30
+nStepmax=6
+nOutput=0
+[mesh]
+nx=14
+ny=12
+nz=16
+boundary_type_xmin=3
+boundary_type_xmax=3
+boundary_type_ymin=1
+boundary_type_ymax=2
+boundary_type_zmin=3
+boundary_type_zmax=3
+[hydro]
+gamma0=1.666
+cfl=0.8
+slope_type=2
+problem=blast
+riemann=hlld
+[blast]
+radius=0.25
+bx_in=0.4
+bz_in=0.7
+bx_out=0.3
+bz_out=0.2
+"""
+
+
+def test_ini_mhd3d_blast_matches_oracle(tmp_path, oracle):
+    ini = tmp_path / "m3.ini"
+    ini.write_text(INI_MHD3D)
+    ekp = make_params("MHD_Muscl_3D", 14, 12, 16, gamma0="1.666", cfl="0.8", slope_type="2",
+                      riemann="hlld", bc=(3, 3, 1, 2, 3, 3))
+    op = oracle.from_ek(ekp)
+    f = lambda s: float(np.float32(s))  # ConfigMap::getFloat
+    U = np.zeros(oracle.ref_shape(op))
+    g = 3
+    ax = [op.xmin + op.dx / 2 + (np.arange(n) - g) * d for n, d in ((op.isize, op.dx), (op.jsize, op.dy),
+                                                                  (op.ksize, op.dz))]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    inside = (X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2 < f("0.25") * f("0.25")
+    a = np.where(inside, f("0.4"), f("0.3"))
+    b = np.where(inside, f("0.7"), f("0.2"))
+    U[..., 0] = np.where(inside, 1.0, f("1.2"))
+    U[..., 5], U[..., 6] = a, b
+    U[..., 1] = np.where(inside, 10.0, f("0.1")) / (op.gamma0 - 1.0) + 0.5 * (a * a + b * b + 0.0)
+    t_o, it_o, _ = oracle.mhd3d_run(op, U, 6, 1e30)
+    G, t, it = ini_run(ini, 8 * 14 * 12 * 16)
+    assert it == it_o == 6 and t == t_o
+    want = U[3:-3, 3:-3, 3:-3, :].transpose(3, 2, 1, 0).reshape(-1)
+    np.testing.assert_array_equal(G, want)

@@ -15,7 +15,7 @@ def small_stream(seed=5, edge=(20, 40)):
 
 
 def alone(j):
-    P = Pool(j.params, 64 << 20, 4)
+    P = Pool(j.params, 64 << 20, 4) if j.kind == "hydro_kh" else serving.ContextGroup(1 << 30)
     try:
         jid = P.admit(j.params, j.U0, j.steps)
         P.step(j.steps + 1)
@@ -33,7 +33,7 @@ def test_each_result_same_alone_and_in_the_mix():
     finally:
         S.close()
     print(f"case=serving mix: {stats}", flush=True)
-    assert stats["jobs"] == 8
+    assert stats["jobs"] == 8 and {j.kind for j in jobs} == {"hydro_kh", "mhd_blast"}
     for j in jobs:
         assert j.done_t >= j.arrival and j.result is not None
         ref = alone(j)
@@ -41,9 +41,13 @@ def test_each_result_same_alone_and_in_the_mix():
 
 
 def test_admission_waits_for_memory():
-    jobs = small_stream(seed=9, edge=(32, 32))
+    jobs = [j for j in small_stream(seed=9, edge=(32, 32))]
+    for j in jobs:  # one kind, so that the budget is one pool's
+        j.kind = "hydro_kh"
+        j.params = serving.job_params("hydro_kh", j.n, native.EK_FP32)
+        j.U0 = serving.job_state(j)
     biggest = max(2 * j.cells * 5 * 4 for j in jobs)
-    S = serving.Server(biggest + (1 << 16))  # room for one job at a time
+    S = serving.Server(biggest + (1 << 16), kinds=("hydro_kh",))  # room for one job at a time
     try:
         stats = S.serve(jobs, keep_results=True)
     finally:
