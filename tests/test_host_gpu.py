@@ -174,8 +174,9 @@ def test_ini_mhd3d_blast_matches_oracle(tmp_path, oracle):
     f = lambda s: float(np.float32(s))  # ConfigMap::getFloat
     U = np.zeros(oracle.ref_shape(op))
     g = 3
-    ax = [op.xmin + op.dx / 2 + (np.arange(n) - g) * d for n, d in ((op.isize, op.dx), (op.jsize, op.dy),
-                                                                  (op.ksize, op.dz))]
+    ax = [lo + d / 2 + (np.arange(n) - g) * d for n, d, lo in ((op.isize, op.dx, op.xmin),
+                                                                 (op.jsize, op.dy, op.ymin),
+                                                                 (op.ksize, op.dz, op.zmin))]
     X, Y, Z = np.meshgrid(*ax, indexing="ij")
     inside = (X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2 < f("0.25") * f("0.25")
     a = np.where(inside, f("0.4"), f("0.3"))
