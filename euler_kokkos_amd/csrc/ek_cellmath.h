@@ -123,19 +123,12 @@ __host__ __device__ inline T slope1(T q, T qp, T qm)
   return (dcen >= T(0)) ? m : -m;
 }
 
-// The 2*DIM face states of one cell (trace_unsplit_{2d,3d}_along_dir,
-// src/muscl/HydroBaseFunctor2D.h:218-293, HydroBaseFunctor3D.h:276-389).
-// fm[d] = state at the cell's max face along d (FACE_XMAX/YMAX/ZMAX),
-// fp[d] = state at its min face (FACE_XMIN/YMIN/ZMIN). dq[d][v] = slope along d.
-// Only the faces selected by `want` (bit 2d = min face, bit 2d+1 = max face) are produced.
+// Source terms of the Hancock half step (the s vector of trace_unsplit_{2d,3d}_along_dir,
+// src/muscl/HydroBaseFunctor2D.h:247-251, HydroBaseFunctor3D.h:317-321).
 template <class T, int DIM>
-__host__ __device__ inline void trace_cell(const Gas<T>& g, const T* q, const T (*dq)[5],
-                                           const T* dtdx, unsigned want, T (*fm)[5], T (*fp)[5])
+__host__ __device__ inline void trace_source(const Gas<T>& g, const T* q, const T (*dq)[5], T* s)
 {
-  constexpr int NV = DIM == 3 ? 5 : 4;
-  const T h = kHalf<T>();
   const T r = q[ID], p = q[IP], u = q[IU], v = q[IV];
-  T s[5];
   if constexpr (DIM == 2) {
     const T drx = dq[0][ID], dpx = dq[0][IP], dux = dq[0][IU], dvx = dq[0][IV];
     const T dry = dq[1][ID], dpy = dq[1][IP], duy = dq[1][IU], dvy = dq[1][IV];
@@ -154,23 +147,42 @@ __host__ __device__ inline void trace_cell(const Gas<T>& g, const T* q, const T 
     s[IV] = -u * dvx - v * dvy - w * dvz - (dpy) / r;
     s[IW] = -u * dwx - v * dwy - w * dwz - (dpz) / r;
   }
+}
+
+// One face state of a cell: maxface -> q + dq/2 + s dt/(2dx) (FACE_XMAX/YMAX/ZMAX), else
+// q - dq/2 + s dt/(2dx) (FACE_*MIN); density floored at smallr.
+template <class T, int NV>
+__host__ __device__ inline void trace_face(const Gas<T>& g, const T* q, const T* dqd, const T* s, T dtdxd,
+                                           bool maxface, T* out)
+{
+  const T h = kHalf<T>();
+#pragma unroll
+  for (int vv = 0; vv < NV; vv++) {
+    const T hd = h * dqd[vv];
+    const T st = s[vv] * dtdxd * h;
+    out[vv] = maxface ? q[vv] + hd + st : q[vv] - hd + st;
+  }
+  out[ID] = emax(g.smallr, out[ID]);
+}
+
+// The 2*DIM face states of one cell (trace_unsplit_{2d,3d}_along_dir,
+// src/muscl/HydroBaseFunctor2D.h:218-293, HydroBaseFunctor3D.h:276-389).
+// fm[d] = state at the cell's max face along d (FACE_XMAX/YMAX/ZMAX),
+// fp[d] = state at its min face (FACE_XMIN/YMIN/ZMIN). dq[d][v] = slope along d.
+// Only the faces selected by `want` (bit 2d = min face, bit 2d+1 = max face) are produced.
+template <class T, int DIM>
+__host__ __device__ inline void trace_cell(const Gas<T>& g, const T* q, const T (*dq)[5],
+                                           const T* dtdx, unsigned want, T (*fm)[5], T (*fp)[5])
+{
+  constexpr int NV = DIM == 3 ? 5 : 4;
+  T s[5];
+  trace_source<T, DIM>(g, q, dq, s);
 #pragma unroll
   for (int d = 0; d < DIM; d++) {
-    if (!(want & (3u << (2 * d))))
-      continue;
-#pragma unroll
-    for (int vv = 0; vv < NV; vv++) {
-      const T hd = h * dq[d][vv];
-      const T st = s[vv] * dtdx[d] * h;
-      if (want & (1u << (2 * d)))
-        fp[d][vv] = q[vv] - hd + st;
-      if (want & (2u << (2 * d)))
-        fm[d][vv] = q[vv] + hd + st;
-    }
     if (want & (1u << (2 * d)))
-      fp[d][ID] = emax(g.smallr, fp[d][ID]);
+      trace_face<T, NV>(g, q, dq[d], s, dtdx[d], false, fp[d]);
     if (want & (2u << (2 * d)))
-      fm[d][ID] = emax(g.smallr, fm[d][ID]);
+      trace_face<T, NV>(g, q, dq[d], s, dtdx[d], true, fm[d]);
   }
 }
 

@@ -222,17 +222,11 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
 
   for (int p = k0 - 1; p <= k1; p++) {
     const bool warm = p == k0 - 1, last = p == k1;
-    // A: primitives of plane p+1 for this column; prefetch p+2; the owner's U(p) for D
+    // A: primitives of plane p+1 for this column; prefetch p+2
     if (haspos) {
       prims<T, 3>(gas, upre, qn);
       if (p + 2 <= k1 + 1)
         fetch(p + 2, upre);
-    }
-    if (ovalid && !warm && !last) {
-      const long long o = (long long)p * geo.plane + ocol;
-#pragma unroll
-      for (int v = 0; v < NV; v++)
-        ucur[v] = Uin[v * geo.vstride + o];
     }
     // B: slopes and face states of plane p
     if (want) {
@@ -240,7 +234,7 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
       if (warm || last)
         w = owner ? (warm ? 0x20u : 0x10u) : 0u;
       if (w) {
-        T dq[3][5], fm[3][5], fp[3][5];
+        T dq[3][5];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
           if constexpr (ST == 0) {
@@ -253,46 +247,54 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
             dq[1][v] = slope1<T, ST>(qc[v], sQ[v][c + QX], sQ[v][c - QX]);
           }
         }
-        trace_cell<T, 3>(gas, qc, dq, dtdx, w, fm, fp);
+        T s[NV];
+        trace_source<T, 3>(gas, qc, dq, s);
         const int cx = qx - 2, cy = qy - 2;
         // rotated stores: component IU <- normal velocity of the face direction
         if (w & 0x2) {
-          const int f = cy * (TX + 1) + cx + 1;
+          T f[NV];
+          trace_face<T, NV>(gas, qc, dq[0], s, dtdx[0], true, f);
+          const int fi = cy * (TX + 1) + cx + 1;
 #pragma unroll
           for (int v = 0; v < NV; v++)
-            sF[0][v][f] = fm[0][v];
+            sF[0][v][fi] = f[v];
         }
         if (w & 0x1) {
-          const int f = cy * (TX + 1) + cx;
+          T f[NV];
+          trace_face<T, NV>(gas, qc, dq[0], s, dtdx[0], false, f);
+          const int fi = cy * (TX + 1) + cx;
 #pragma unroll
           for (int v = 0; v < NV; v++)
-            sF[1][v][f] = fp[0][v];
+            sF[1][v][fi] = f[v];
         }
         if (w & 0x8) {
-          const int f = NFX + (cy + 1) * TX + cx;
+          T f[NV];
+          trace_face<T, NV>(gas, qc, dq[1], s, dtdx[1], true, f);
+          const int fi = NFX + (cy + 1) * TX + cx;
 #pragma unroll
           for (int v = 0; v < NV; v++)
-            sF[0][v][f] = fm[1][rot_y(v)];
+            sF[0][v][fi] = f[rot_y(v)];
         }
         if (w & 0x4) {
-          const int f = NFX + cy * TX + cx;
+          T f[NV];
+          trace_face<T, NV>(gas, qc, dq[1], s, dtdx[1], false, f);
+          const int fi = NFX + cy * TX + cx;
 #pragma unroll
           for (int v = 0; v < NV; v++)
-            sF[1][v][f] = fp[1][rot_y(v)];
+            sF[1][v][fi] = f[rot_y(v)];
         }
         if (w & 0x10) {
-          const int f = NFX + NFY + tid;
+          T f[NV];
+          trace_face<T, NV>(gas, qc, dq[2], s, dtdx[2], false, f);
+          const int fi = NFX + NFY + tid;
 #pragma unroll
           for (int v = 0; v < NV; v++) {
-            sF[0][v][f] = qzm[rot_z(v)];
-            sF[1][v][f] = fp[2][rot_z(v)];
+            sF[0][v][fi] = qzm[rot_z(v)];
+            sF[1][v][fi] = f[rot_z(v)];
           }
         }
-        if (w & 0x20) {
-#pragma unroll
-          for (int v = 0; v < NV; v++)
-            qzm[v] = fm[2][v];
-        }
+        if (w & 0x20)
+          trace_face<T, NV>(gas, qc, dq[2], s, dtdx[2], true, qzm);
       }
     }
     __syncthreads();
@@ -347,6 +349,14 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
         }
       }
       if (!last) {
+        // the owner's U(p), read late to keep it out of the register peak of B and C (this
+        // thread streamed the same column two planes ago, so the read hits in cache)
+        if (ovalid) {
+          const long long o = (long long)p * geo.plane + ocol;
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            ucur[v] = Uin[v * geo.vstride + o];
+        }
 #pragma unroll
         for (int v = 0; v < NV; v++) {
           T a = ovalid ? ucur[v] : T(0);

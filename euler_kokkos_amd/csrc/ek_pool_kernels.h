@@ -77,7 +77,7 @@ __global__ __launch_bounds__(NT) void hydro2d_pool_step(const T* __restrict__ Ua
 }
 
 template <class T, int TX, int TY, int NT, int RS, int ST>
-__global__ __launch_bounds__(NT) void hydro3d_pool_step(const T* __restrict__ Ua, T* __restrict__ Ub,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void hydro3d_pool_step(const T* __restrict__ Ua, T* __restrict__ Ub,
                                                         const PoolJob<T>* __restrict__ jobs,
                                                         const PoolWork* __restrict__ work, int nwork,
                                                         TimeSlot* ts, int slot)
