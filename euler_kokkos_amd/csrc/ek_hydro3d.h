@@ -8,12 +8,19 @@
 //
 // Per plane p:  A  load U(p+1), primitives into registers (all positions)
 //               B  slopes + Hancock trace at the tile and its 1-wide halo cross;
-//                  face states to LDS (x, y, and the tile's z faces)
+//                  x/y face states to LDS; each tile cell parks its two z-face states
+//                  (max-z of plane p-1, min-z of plane p) in its own LDS slot
+//               -- barrier --
 //               W  primitives of plane p+1 into the LDS plane
-//               C  Riemann problems: (TX+1)TY x-faces, TX(TY+1) y-faces, TXTY z-faces,
-//                  spread over the waves so that each wave's total work is even
+//               C  Riemann problems: every tile cell solves its own z face (in place, its
+//                  slot only); the (TX+1)TY x-faces and TX(TY+1) y-faces are split over the
+//                  waves so that each wave's share of this phase is even; x/y fluxes go to
+//                  their own buffer
+//               -- barrier --
 //               D  update of plane p (UpdateFunctor3D order) and the -Fz(p) term plus the
 //                  CFL contribution of plane p-1
+// D reads only the x/y flux buffer (rewritten after the next plane's first barrier) and
+// the cell's own z slot, so no third barrier is needed: D of plane p overlaps A/B of p+1.
 // The arithmetic per cell, face and update is exactly the reference's
 // (src/muscl/HydroRunFunctors3D.h:271-820); only where and how often it is evaluated
 // changes: slopes once per cell (the reference recomputes them for 4 cells), each face
@@ -31,15 +38,18 @@ struct Col3 {
   static constexpr int PH = 2 * TX + 2 * TY;       // halo cross (one face state each)
   static constexpr int PO = 2 * TX + 2 * TY + 4;   // primitives only (slope stencils)
   static constexpr int NPOS = PT + PH + PO;
-  static constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1), NFZ = PT;
-  static constexpr int NF = NFX + NFY + NFZ;
+  static constexpr int NFX = (TX + 1) * TY, NFY = TX * (TY + 1);
+  static constexpr int NXY = NFX + NFY;            // x and y faces (shared list)
+  static constexpr int NF = NXY + PT;              // plus one z slot per tile cell
   static constexpr int NW = NT / 64;
   static_assert(NT % 64 == 0, "whole waves");
   static_assert(NPOS <= NT, "one primitive column per thread");
 };
 
-// Static split of the face list over the waves: each 64-face round goes to the wave whose
-// estimated work (slopes/trace of its lanes, update, rounds already given) is smallest.
+// Static split of the x/y face list over the waves. Tile-cell waves already solve their
+// own z faces in the Riemann phase (one round); each further 64-face round goes to the
+// wave with the fewest rounds in that phase, ties to the one with less work before it
+// (trace, update), so the phase between the two barriers is as short as it can be.
 template <int TX, int TY, int NT>
 struct FacePlan {
   int start[NT / 64];
@@ -51,34 +61,44 @@ constexpr FacePlan<TX, TY, NT> make_face_plan()
 {
   using C = Col3<TX, TY, NT>;
   FacePlan<TX, TY, NT> fp{};
-  int load[C::NW] = {};
-  int rounds[C::NW] = {};
+  int before[C::NW] = {};  // work before the first barrier: update of p-1, trace of p
+  int rounds[C::NW] = {};  // 64-face rounds between the barriers
+  int xy[C::NW] = {};
   for (int w = 0; w < C::NW; w++) {
     const int lo = 64 * w, hi = 64 * w + 64;
     const bool tile = lo < C::PT;
     const bool cross = hi > C::PT && lo < C::PT + C::PH;
-    load[w] = (tile ? 34 + 15 : (cross ? 29 : 0));
+    before[w] = tile ? 34 + 15 : (cross ? 29 : 0);
+    rounds[w] = tile ? 1 : 0;
   }
-  const int R = (C::NF + 63) / 64;
+  const int R = (C::NXY + 63) / 64;
   for (int r = 0; r < R; r++) {
     int best = 0;
     for (int w = 1; w < C::NW; w++)
-      if (load[w] < load[best])
+      if (rounds[w] < rounds[best] || (rounds[w] == rounds[best] && before[w] < before[best]))
         best = w;
-    load[best] += 20;
     rounds[best] += 1;
+    xy[best] += 1;
   }
-  // contiguous ranges, lightest-loaded waves first so the partial round lands last
+  // contiguous ranges; the wave with most work before the barrier goes last and so takes
+  // the partial round
   int order[C::NW] = {};
   for (int w = 0; w < C::NW; w++)
-    order[w] = C::NW - 1 - w;
+    order[w] = w;
+  for (int i = 0; i < C::NW; i++)
+    for (int j = i + 1; j < C::NW; j++)
+      if (before[order[j]] < before[order[i]]) {
+        const int t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+      }
   int s = 0;
   for (int i = 0; i < C::NW; i++) {
     const int w = order[i];
     fp.start[w] = s;
-    int c = rounds[w] * 64;
-    if (s + c > C::NF)
-      c = C::NF - s;
+    int c = xy[w] * 64;
+    if (s + c > C::NXY)
+      c = C::NXY - s;
     if (c < 0)
       c = 0;
     fp.count[w] = c;
@@ -156,14 +176,17 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
   using C = Col3<TX, TY, NT>;
   constexpr int NV = 5;
   constexpr int QX = C::QX, QN = C::QN, PT = C::PT;
-  constexpr int NFX = C::NFX, NFY = C::NFY, NF = C::NF;
+  constexpr int NFX = C::NFX, NXY = C::NXY, NF = C::NF;
   constexpr FacePlan<TX, TY, NT> plan = make_face_plan<TX, TY, NT>();
   __shared__ T sQ[NV][QN];
   // Face states, rotated so that IU holds the normal velocity (the swap the reference does
   // before each y/z solve, HydroRunFunctors3D.h:597,692): [0] left state, [1] right state.
-  // Faces: x (fy*(TX+1)+fx), then y (NFX + fy*TX+fx), then z (NFX+NFY + tile cell).
-  // After the solve, [0][.][f] holds the face flux times dt/dx, still rotated.
+  // Faces: x (fy*(TX+1)+fx), then y (NFX + fy*TX+fx), then z (NXY + tile cell). A z slot
+  // is touched only by its own tile cell's thread; after its solve, [0][.][z] holds the
+  // z flux times dt/dz, still rotated.
   __shared__ T sF[2][NV][NF];
+  // x/y fluxes times dt/dx (dt/dy), rotated like the states
+  __shared__ T sG[NV][NXY];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -286,7 +309,7 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
         if (w & 0x10) {
           T f[NV];
           trace_face<T, NV>(gas, qc, dq[2], s, dtdx[2], false, f);
-          const int fi = NFX + NFY + tid;
+          const int fi = NXY + tid;
 #pragma unroll
           for (int v = 0; v < NV; v++) {
             sF[0][v][fi] = qzm[rot_z(v)];
@@ -304,18 +327,10 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
       for (int v = 0; v < NV; v++)
         sQ[v][c] = qn[v];
     }
-    // C: Riemann problems of plane p's faces
+    // C: Riemann problems of plane p's faces: own z face, then this wave's x/y range
     if (!warm) {
-      int fb, fe;
-      if (last) {
-        fb = NFX + NFY + tid;
-        fe = NF;
-      } else {
-        fb = plan.start[wave] + lane;
-        fe = plan.start[wave] + plan.count[wave];
-      }
-      const int stride = last ? NT : 64;
-      for (int f = fb; f < fe; f += stride) {
+      if (owner) {
+        const int f = NXY + tid;
         T ql[NV], qr[NV], fl[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
@@ -323,10 +338,25 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
           qr[v] = sF[1][v][f];
         }
         riemann<T, NV, RS>(gas, ql, qr, fl);
-        const T dtd = f < NFX ? dtdx[0] : (f < NFX + NFY ? dtdx[1] : dtdx[2]);
 #pragma unroll
         for (int v = 0; v < NV; v++)
-          sF[0][v][f] = fl[v] * dtd;
+          sF[0][v][f] = fl[v] * dtdx[2];
+      }
+      if (!last) {
+        const int fe = plan.start[wave] + plan.count[wave];
+        for (int f = plan.start[wave] + lane; f < fe; f += 64) {
+          T ql[NV], qr[NV], fl[NV];
+#pragma unroll
+          for (int v = 0; v < NV; v++) {
+            ql[v] = sF[0][v][f];
+            qr[v] = sF[1][v][f];
+          }
+          riemann<T, NV, RS>(gas, ql, qr, fl);
+          const T dtd = f < NFX ? dtdx[0] : dtdx[1];
+#pragma unroll
+          for (int v = 0; v < NV; v++)
+            sG[v][f] = fl[v] * dtd;
+        }
       }
     }
     __syncthreads();
@@ -336,7 +366,7 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
         T u[NV];
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-          acc[v] -= sF[0][rot_z(v)][NFX + NFY + tid];
+          acc[v] -= sF[0][rot_z(v)][NXY + tid];
           u[v] = acc[v];
         }
         if (ovalid) {
@@ -360,16 +390,17 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
 #pragma unroll
         for (int v = 0; v < NV; v++) {
           T a = ovalid ? ucur[v] : T(0);
-          a += sF[0][v][oy * (TX + 1) + ox];
-          a -= sF[0][v][oy * (TX + 1) + ox + 1];
-          a += sF[0][rot_y(v)][NFX + oy * TX + ox];
-          a -= sF[0][rot_y(v)][NFX + (oy + 1) * TX + ox];
-          a += sF[0][rot_z(v)][NFX + NFY + tid];
+          a += sG[v][oy * (TX + 1) + ox];
+          a -= sG[v][oy * (TX + 1) + ox + 1];
+          a += sG[rot_y(v)][NFX + oy * TX + ox];
+          a -= sG[rot_y(v)][NFX + (oy + 1) * TX + ox];
+          a += sF[0][rot_z(v)][NXY + tid];
           acc[v] = a;
         }
       }
     }
-    __syncthreads();
+    // no barrier here: D reads sG (next written after the next plane's first barrier) and
+    // this thread's own z slot
 #pragma unroll
     for (int v = 0; v < NV; v++) {
       qp[v] = qc[v];
