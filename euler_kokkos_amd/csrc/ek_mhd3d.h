@@ -431,7 +431,12 @@ __device__ __forceinline__ double cfl3(const Gas<T>& gas, const T* u, T mx, T my
   }
 }
 
-// update on the interior (UpdateFunctor3D_MHD + UpdateEmfFunctor3D) into the SoA output
+// update on the interior (UpdateFunctor3D_MHD + UpdateEmfFunctor3D) into the SoA output.
+// A grid-stride loop over a bounded grid (MHD3_UPDATE_BLOCKS): each block ends with one
+// atomic max on the time slot, and one short block per 256 cells (27k blocks at 192^3)
+// made those atomics cost three quarters of the kernel's time.
+constexpr unsigned MHD3_UPDATE_BLOCKS = 2048;
+
 template <class T>
 __global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T* __restrict__ Uout, Work3 w,
                                                      Geom<T> geo, Gas<T> gas, StepCtl ctl)
@@ -444,9 +449,9 @@ __global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T*
     return;
   const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
   const long long n = geo.vstride;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double local = 0.0;
-  if (c < n) {
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x) {
     const int ii = (int)(c % w.nx), jj = (int)((c / w.nx) % w.ny), kk = (int)(c / ((long long)w.nx * w.ny));
     const long long PN = w.PN, o = pidx(w, ii + 3, jj + 3, kk + 3);
     const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
@@ -504,7 +509,8 @@ __global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T*
     } else {
       cf = geo.bc[5] == BC_DIRICHLET ? u[IC] * T(-1) : u[IC];
     }
-    local = cfl3<T>(gas, u, ar, bt, cf, geo);
+    const double sv = cfl3<T>(gas, u, ar, bt, cf, geo);
+    local = (sv > local || !(sv == sv)) ? sv : local;
   }
   block_max_to<256>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
   if (first_block && threadIdx.x == 0)

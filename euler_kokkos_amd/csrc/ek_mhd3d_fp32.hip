@@ -10,7 +10,8 @@ static hipError_t launch3(const float* Uin, float* Uout, float* S, const mhd3::W
                           const Gas<float>& gas, const StepCtl& ctl, hipStream_t s)
 {
   const unsigned nb = (unsigned)((w.PN + 255) / 256);
-  const unsigned ni = (unsigned)((geo.vstride + 255) / 256);
+  const long long nu = (geo.vstride + 255) / 256;
+  const unsigned ni = (unsigned)(nu < mhd3::MHD3_UPDATE_BLOCKS ? nu : mhd3::MHD3_UPDATE_BLOCKS);
   hipLaunchKernelGGL(mhd3::pad_kernel<float>, dim3(nb), dim3(256), 0, s, Uin, S, w, geo, ctl);
   hipLaunchKernelGGL(mhd3::prims_kernel<float>, dim3(nb), dim3(256), 0, s, S, w, gas, ctl);
   hipLaunchKernelGGL((mhd3::efield_kernel<float, ST>), dim3(nb), dim3(256), 0, s, S, w, ctl);
