@@ -19,6 +19,7 @@ struct ek_ctx {
   int cur = 0;
   ek::TimeSlot* ts = nullptr;
   unsigned long long* scratch_bits = nullptr;
+  unsigned long long* spread = nullptr; // extra CFL words of the three time slots
   double* partial = nullptr;
   double* stage = nullptr;   // FP64 staging buffer for reference-layout transfers
   void* work = nullptr;      // 3D MHD ghost-padded work area
@@ -140,11 +141,12 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
   if (hipMalloc(&c->U[0], state_bytes) != hipSuccess || hipMalloc(&c->U[1], state_bytes) != hipSuccess ||
       hipMalloc(&c->ts, 4 * sizeof(ek::TimeSlot)) != hipSuccess ||
       hipMalloc(&c->scratch_bits, 64) != hipSuccess ||
+      hipMalloc(&c->spread, ek::SPREAD_ELEMS * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->partial, (size_t)c->nblocks_budget * 16 * sizeof(double)) != hipSuccess) {
     ek_destroy(c);
     return fail(nullptr, EK_ENOMEM, "device allocation failed");
   }
-  c->bytes = (long long)(2 * state_bytes + 4 * sizeof(ek::TimeSlot) + 64 +
+  c->bytes = (long long)(2 * state_bytes + 4 * sizeof(ek::TimeSlot) + 64 + ek::SPREAD_ELEMS * 8 +
                          (size_t)c->nblocks_budget * 16 * sizeof(double));
   if (c->p.mhd && c->p.dim == 3) {
     const size_t wb = (size_t)ek::fp64::mhd3_scratch_elems(c->p.nx, c->p.ny, c->p.nz) * c->elem;
@@ -157,6 +159,7 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
   hipMemsetAsync(c->U[0], 0, state_bytes, c->stream);
   hipMemsetAsync(c->U[1], 0, state_bytes, c->stream);
   hipMemsetAsync(c->ts, 0, 4 * sizeof(ek::TimeSlot), c->stream);
+  hipMemsetAsync(c->spread, 0, ek::SPREAD_ELEMS * sizeof(unsigned long long), c->stream);
   if ((e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
       (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
     ek_destroy(c);
@@ -182,6 +185,7 @@ extern "C" void ek_destroy(ek_ctx* c)
   hipFree(c->U[1]);
   hipFree(c->ts);
   hipFree(c->scratch_bits);
+  hipFree(c->spread);
   hipFree(c->partial);
   hipFree(c->stage);
   hipFree(c->work);
@@ -345,6 +349,7 @@ extern "C" int ek_step(ek_ctx* c, double dt)
   EK_HIP(c, hipMemsetAsync(c->ts, 0, 3 * sizeof(ek::TimeSlot), c->stream));
   ek::StepCtl ctl{};
   ctl.ts = c->ts;
+  ctl.spread = c->spread;
   ctl.slot = 0;
   ctl.mode = 1;
   ctl.dt_given = dt;
@@ -367,6 +372,7 @@ static int build_graph(ek_ctx* c, int nStepmax, double tEnd)
   for (int i = 0; i < 6; i++) {
     ek::StepCtl ctl{};
     ctl.ts = c->ts;
+    ctl.spread = c->spread;
     ctl.slot = i % 3;
     ctl.mode = 0;
     ctl.cfl = c->p.cfl;
@@ -408,6 +414,7 @@ extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* 
   init[0].t = *t_io;
   init[0].it = *it_io;
   EK_HIP(c, hipMemcpyAsync(c->ts, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  EK_HIP(c, hipMemsetAsync(c->spread, 0, ek::SPREAD_ELEMS * sizeof(unsigned long long), c->stream));
   EK_HIP(c, launch_inv_dt_into(c, &c->ts[0].invdt_bits));
   const int nfull = nsteps / 6, rem = nsteps % 6;
   if (nfull > 0 && (!c->graph || c->graph_nStepmax != nStepmax || c->graph_tEnd != tEnd ||
@@ -422,6 +429,7 @@ extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* 
   for (int r = 0; r < rem; r++) {
     ek::StepCtl ctl{};
     ctl.ts = c->ts;
+    ctl.spread = c->spread;
     ctl.slot = r % 3;
     ctl.mode = 0;
     ctl.cfl = c->p.cfl;

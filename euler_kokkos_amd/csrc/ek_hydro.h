@@ -31,6 +31,7 @@ struct TimeSlot {
 
 struct StepCtl {
   TimeSlot* ts;
+  unsigned long long* spread; // extra CFL words of the slots (see invdt_word), or null
   int slot;        // slot this launch reads
   int mode;        // 0: dt = cfl/invdt (clamped to tEnd), 1: dt = dt_given
   double dt_given;
@@ -88,12 +89,13 @@ __device__ __forceinline__ unsigned long long double_to_bits(double d)
   return (unsigned long long)__double_as_longlong(d);
 }
 
-// Wave-wide max of a non-negative double with DPP row shifts and row broadcasts
-// (GFX9 DPP), result valid in lane 63 and returned via readlane.
-__device__ __forceinline__ double dpp_max_shr(double v, int ctrl_sel)
+// Wave-wide maxima with DPP row shifts and row broadcasts (GFX9 DPP), result valid in
+// lane 63 and returned via readlane. dpp_shr_bits moves a 64-bit value by one step of the
+// pattern; lanes with no source read 0, the identity of both maxima below.
+__device__ __forceinline__ unsigned long long dpp_shr_bits(unsigned long long v, int ctrl_sel)
 {
-  int lo = __double2loint(v);
-  int hi = __double2hiint(v);
+  const int lo = (int)(unsigned)v;
+  const int hi = (int)(unsigned)(v >> 32);
   int lo2, hi2;
   switch (ctrl_sel) {
     case 1:
@@ -121,9 +123,34 @@ __device__ __forceinline__ double dpp_max_shr(double v, int ctrl_sel)
       hi2 = __builtin_amdgcn_update_dpp(0, hi, 0x143, 0xc, 0xf, false);
       break;
   }
-  const double o = __hiloint2double(hi2, lo2);
-  // values are >= 0; lanes with no source read 0, the identity of this max
+  return ((unsigned long long)(unsigned)hi2 << 32) | (unsigned)lo2;
+}
+
+__device__ __forceinline__ double dpp_max_shr(double v, int ctrl_sel)
+{
+  const double o = bits_to_double(dpp_shr_bits(double_to_bits(v), ctrl_sel));
+  // values are >= 0
   return v > o ? v : (o == o ? o : v);
+}
+
+__device__ __forceinline__ unsigned long long wave_umax(unsigned long long v)
+{
+  unsigned long long o;
+  o = dpp_shr_bits(v, 1);
+  v = o > v ? o : v;
+  o = dpp_shr_bits(v, 2);
+  v = o > v ? o : v;
+  o = dpp_shr_bits(v, 4);
+  v = o > v ? o : v;
+  o = dpp_shr_bits(v, 8);
+  v = o > v ? o : v;
+  o = dpp_shr_bits(v, 15);
+  v = o > v ? o : v;
+  o = dpp_shr_bits(v, 31);
+  v = o > v ? o : v;
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+  return ((unsigned long long)hi << 32) | lo;
 }
 
 __device__ __forceinline__ double wave_max(double v)
@@ -137,6 +164,37 @@ __device__ __forceinline__ double wave_max(double v)
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
   return __hiloint2double(hi, lo);
+}
+
+// CFL words of a time slot. Every block of a step ends with one device-scope atomic max,
+// and such atomics on one address serialize: with thousands of blocks per launch that cost
+// up to three quarters of a kernel's time. With StepCtl::spread set, the maximum of slot s
+// lives in NSPREAD words 256 bytes apart -- ts[s].invdt_bits (key 0) and
+// spread[(s*NSPREAD + k)*SPREAD_STRIDE] for k = 1..NSPREAD-1 -- block b updates word
+// b % NSPREAD and readers take the max of all of them. Max is exact, so dt is unchanged.
+constexpr int NSPREAD = 64;
+constexpr int SPREAD_STRIDE = 32;
+constexpr long long SPREAD_ELEMS = 3LL * NSPREAD * SPREAD_STRIDE;
+
+__device__ __forceinline__ unsigned long long* invdt_word(const StepCtl& c, int slot, unsigned key)
+{
+  const unsigned k = key & (NSPREAD - 1);
+  if (!c.spread || k == 0)
+    return &c.ts[slot].invdt_bits;
+  return c.spread + ((long long)slot * NSPREAD + k) * SPREAD_STRIDE;
+}
+
+// Max over the words of `slot`; `own` is ts[slot].invdt_bits as already read. Call with
+// the whole wave active (lane k reads word k).
+__device__ __forceinline__ unsigned long long invdt_gather(const StepCtl& c, int slot, unsigned long long own)
+{
+  static_assert(NSPREAD == 64, "one word per lane");
+  if (!c.spread)
+    return own;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long v =
+      lane == 0 ? own : c.spread[((long long)slot * NSPREAD + lane) * SPREAD_STRIDE];
+  return wave_umax(v);
 }
 
 // Block max then one atomic per block. NaN (a failed cell) wins: its bit pattern is above
@@ -182,7 +240,7 @@ __device__ __forceinline__ bool step_begin(const StepCtl& c, bool first_block, d
       }
       return false;
     }
-    dt = c.cfl / bits_to_double(cur.invdt_bits);
+    dt = c.cfl / bits_to_double(invdt_gather(c, c.slot, cur.invdt_bits));
     if (cur.t + dt > c.tEnd)
       dt = c.tEnd - cur.t;
   } else {
@@ -191,15 +249,22 @@ __device__ __forceinline__ bool step_begin(const StepCtl& c, bool first_block, d
   return true;
 }
 
+// Called by the first wave of the step's first block: lane 0 writes the next slot's time
+// and clears the slot after it; the other lanes clear that slot's spread words.
 __device__ __forceinline__ void step_end_bookkeeping(const StepCtl& c, const TimeSlot& cur, double dt)
 {
   const int nxt = (c.slot + 1) % 3;
   const int nn = (c.slot + 2) % 3;
-  c.ts[nxt].t = cur.t + dt;
-  c.ts[nxt].dt = dt;
-  c.ts[nxt].it = cur.it + 1;
-  c.ts[nxt].finished = 0;
-  c.ts[nn].invdt_bits = 0ull;
+  const int lane = threadIdx.x & 63;
+  if (lane == 0) {
+    c.ts[nxt].t = cur.t + dt;
+    c.ts[nxt].dt = dt;
+    c.ts[nxt].it = cur.it + 1;
+    c.ts[nxt].finished = 0;
+    c.ts[nn].invdt_bits = 0ull;
+  } else if (c.spread) {
+    c.spread[((long long)nn * NSPREAD + lane) * SPREAD_STRIDE] = 0ull;
+  }
 }
 
 // CFL term of one cell of the new state. The FP64 state uses the reference's expression
@@ -403,8 +468,8 @@ __global__ __launch_bounds__(NT) void hydro2d_step(const T* __restrict__ Uin, T*
     return;
   const double local =
       hydro2d_tile<T, TX, TY, NT, RS, ST>(Uin, Uout, geo, gas, dt, blockIdx.x * TX, blockIdx.y * TY);
-  block_max_to<NT>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
-  if (first_block && threadIdx.x == 0)
+  block_max_to<NT>(invdt_word(ctl, (ctl.slot + 1) % 3, blockIdx.x + blockIdx.y * gridDim.x), local, sred);
+  if (first_block && threadIdx.x < 64)
     step_end_bookkeeping(ctl, cur, dt);
 }
 

@@ -430,8 +430,8 @@ __global__ __launch_bounds__(NT) void hydro3d_step(const T* __restrict__ Uin, T*
     return;
   const double local =
       hydro3d_column<T, TX, TY, NT, RS, ST>(Uin, Uout, geo, gas, dt, bx * TX, by * TY, bz * CZ, CZ);
-  block_max_to<NT>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
-  if (first_block && threadIdx.x == 0)
+  block_max_to<NT>(invdt_word(ctl, (ctl.slot + 1) % 3, b), local, sred);
+  if (first_block && threadIdx.x < 64)
     step_end_bookkeeping(ctl, cur, dt);
 }
 

@@ -866,8 +866,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 2))) void
       local = (s > local || !(s == s)) ? s : local;
     }
   }
-  block_max_to<NT>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
-  if (first_block && tid == 0)
+  block_max_to<NT>(invdt_word(ctl, (ctl.slot + 1) % 3, blockIdx.x + blockIdx.y * gridDim.x), local, sred);
+  if (first_block && tid < 64)
     step_end_bookkeeping(ctl, cur, dt);
 }
 

@@ -67,7 +67,7 @@ __device__ __forceinline__ bool step_dt(const StepCtl& c, double& dt)
   if (c.mode == 0) {
     if (cur.finished || cur.t >= (c.tEnd - 1e-14) || cur.it >= c.nStepmax)
       return false;
-    dt = c.cfl / bits_to_double(cur.invdt_bits);
+    dt = c.cfl / bits_to_double(invdt_gather(c, c.slot, cur.invdt_bits));
     if (cur.t + dt > c.tEnd)
       dt = c.tEnd - cur.t;
   } else {
@@ -512,8 +512,8 @@ __global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T*
     const double sv = cfl3<T>(gas, u, ar, bt, cf, geo);
     local = (sv > local || !(sv == sv)) ? sv : local;
   }
-  block_max_to<256>(&ctl.ts[(ctl.slot + 1) % 3].invdt_bits, local, sred);
-  if (first_block && threadIdx.x == 0)
+  block_max_to<256>(invdt_word(ctl, (ctl.slot + 1) % 3, blockIdx.x), local, sred);
+  if (first_block && threadIdx.x < 64)
     step_end_bookkeeping(ctl, cur, dt);
 }
 

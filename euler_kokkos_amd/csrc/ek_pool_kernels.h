@@ -56,6 +56,7 @@ __global__ __launch_bounds__(NT) void hydro2d_pool_step(const T* __restrict__ Ua
     const PoolJob<T> J = jobs[it.job];
     StepCtl ctl;
     ctl.ts = ts + J.slot0;
+    ctl.spread = nullptr; // jobs' atomics already land on different slots
     ctl.slot = slot;
     ctl.mode = 0;
     ctl.dt_given = 0.0;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(NT) void hydro2d_pool_step(const T* __restrict__ Ua
     const double local =
         hydro2d_tile<T, TX, TY, NT, RS, ST>(Ua + J.off, Ub + J.off, J.geo, J.gas, dt, it.tx * TX, it.ty * TY);
     block_max_to<NT>(&ctl.ts[(slot + 1) % 3].invdt_bits, local, sred);
-    if (first && threadIdx.x == 0)
+    if (first && threadIdx.x < 64)
       step_end_bookkeeping(ctl, cur, dt);
   }
 }
@@ -90,6 +91,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void hy
     const PoolJob<T> J = jobs[it.job];
     StepCtl ctl;
     ctl.ts = ts + J.slot0;
+    ctl.spread = nullptr; // jobs' atomics already land on different slots
     ctl.slot = slot;
     ctl.mode = 0;
     ctl.dt_given = 0.0;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void hy
                                                                it.tx * TX, it.ty * TY, (it.tz_first >> 1) * J.cz,
                                                                J.cz);
     block_max_to<NT>(&ctl.ts[(slot + 1) % 3].invdt_bits, local, sred);
-    if (first && threadIdx.x == 0)
+    if (first && threadIdx.x < 64)
       step_end_bookkeeping(ctl, cur, dt);
   }
 }
