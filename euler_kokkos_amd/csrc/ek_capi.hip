@@ -4,6 +4,7 @@
 #include "ek_internal.h"
 #include "ek_geom_host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -97,6 +98,11 @@ int ek::validate_params(const ek_params* p, std::string& why)
       return why = "the MHD trace supports slope_type 1 and 2 only", 1;
     if (p->cIso > 0)
       return why = "isothermal MHD (cIso > 0) is not provided", 1;
+    // the 3D pipeline addresses each work-area array with 32-bit byte offsets
+    if (p->dim == 3 &&
+        (double)(p->nx + 6) * (double)(p->ny + 6) * (double)(p->nz + 6) * (p->precision == EK_FP64 ? 8.0 : 4.0) >=
+            4294967296.0)
+      return why = "3D MHD grid too large: each padded array must stay below 4 GiB", 1;
     return validate_common(p, why);
   }
   if (p->gw != 2)
@@ -140,13 +146,14 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
   c->nblocks_budget = (int)std::min<long long>(1024, (c->ncell + 255) / 256);
   if (hipMalloc(&c->U[0], state_bytes) != hipSuccess || hipMalloc(&c->U[1], state_bytes) != hipSuccess ||
       hipMalloc(&c->ts, 4 * sizeof(ek::TimeSlot)) != hipSuccess ||
-      hipMalloc(&c->scratch_bits, 64) != hipSuccess ||
+      hipMalloc(&c->scratch_bits, (size_t)ek::NSPREAD * ek::SPREAD_STRIDE * 8) != hipSuccess ||
       hipMalloc(&c->spread, ek::SPREAD_ELEMS * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->partial, (size_t)c->nblocks_budget * 16 * sizeof(double)) != hipSuccess) {
     ek_destroy(c);
     return fail(nullptr, EK_ENOMEM, "device allocation failed");
   }
-  c->bytes = (long long)(2 * state_bytes + 4 * sizeof(ek::TimeSlot) + 64 + ek::SPREAD_ELEMS * 8 +
+  c->bytes = (long long)(2 * state_bytes + 4 * sizeof(ek::TimeSlot) + ek::NSPREAD * ek::SPREAD_STRIDE * 8 +
+                         ek::SPREAD_ELEMS * 8 +
                          (size_t)c->nblocks_budget * 16 * sizeof(double));
   if (c->p.mhd && c->p.dim == 3) {
     const size_t wb = (size_t)ek::fp64::mhd3_scratch_elems(c->p.nx, c->p.ny, c->p.nz) * c->elem;
@@ -289,32 +296,38 @@ extern "C" int ek_download(ek_ctx* c, double* host, int layout)
   return EK_OK;
 }
 
-static hipError_t launch_inv_dt_into(ek_ctx* c, unsigned long long* dst)
+// CFL reduction of the current state into dst; with spread set, blocks also use the words
+// spread + k*SPREAD_STRIDE (k = 1..63) and the result is the max of dst and those words.
+static hipError_t launch_inv_dt_into(ek_ctx* c, unsigned long long* dst, unsigned long long* spread)
 {
   if (c->p.mhd && c->p.dim == 3) {
     if (c->p.precision == EK_FP64)
-      return ek::fp64::mhd3_inv_dt((const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
-    return ek::fp32::mhd3_inv_dt((const float*)c->U[c->cur], c->g32, c->gas32, dst, c->stream);
+      return ek::fp64::mhd3_inv_dt((const double*)c->U[c->cur], c->g64, c->gas64, dst, spread, c->stream);
+    return ek::fp32::mhd3_inv_dt((const float*)c->U[c->cur], c->g32, c->gas32, dst, spread, c->stream);
   }
   if (c->p.mhd) {
     if (c->p.precision == EK_FP64)
-      return ek::fp64::mhd_inv_dt((const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
-    return ek::fp32::mhd_inv_dt((const float*)c->U[c->cur], c->g32, c->gas32, dst, c->stream);
+      return ek::fp64::mhd_inv_dt((const double*)c->U[c->cur], c->g64, c->gas64, dst, spread, c->stream);
+    return ek::fp32::mhd_inv_dt((const float*)c->U[c->cur], c->g32, c->gas32, dst, spread, c->stream);
   }
   if (c->p.precision == EK_FP64)
-    return ek::fp64::inv_dt(c->p.dim, (const double*)c->U[c->cur], c->g64, c->gas64, dst, c->stream);
-  return ek::fp32::inv_dt(c->p.dim, (const float*)c->U[c->cur], c->g32, c->gas32, dst, c->stream);
+    return ek::fp64::inv_dt(c->p.dim, (const double*)c->U[c->cur], c->g64, c->gas64, dst, spread, c->stream);
+  return ek::fp32::inv_dt(c->p.dim, (const float*)c->U[c->cur], c->g32, c->gas32, dst, spread, c->stream);
 }
 
 extern "C" int ek_inv_dt(ek_ctx* c, double* inv_dt)
 {
   if (!c || !inv_dt)
     return fail(c, EK_EINVAL, "null argument");
-  EK_HIP(c, hipMemsetAsync(c->scratch_bits, 0, 8, c->stream));
-  EK_HIP(c, launch_inv_dt_into(c, c->scratch_bits));
-  unsigned long long b = 0;
-  EK_HIP(c, hipMemcpyAsync(&b, c->scratch_bits, 8, hipMemcpyDeviceToHost, c->stream));
+  const size_t nb = (size_t)ek::NSPREAD * ek::SPREAD_STRIDE * sizeof(unsigned long long);
+  EK_HIP(c, hipMemsetAsync(c->scratch_bits, 0, nb, c->stream));
+  EK_HIP(c, launch_inv_dt_into(c, c->scratch_bits, c->scratch_bits));
+  std::vector<unsigned long long> w((size_t)ek::NSPREAD * ek::SPREAD_STRIDE);
+  EK_HIP(c, hipMemcpyAsync(w.data(), c->scratch_bits, nb, hipMemcpyDeviceToHost, c->stream));
   EK_HIP(c, hipStreamSynchronize(c->stream));
+  unsigned long long b = 0;
+  for (int k = 0; k < ek::NSPREAD; k++)
+    b = std::max(b, w[(size_t)k * ek::SPREAD_STRIDE]);
   std::memcpy(inv_dt, &b, 8);
   return EK_OK;
 }
@@ -415,7 +428,7 @@ extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* 
   init[0].it = *it_io;
   EK_HIP(c, hipMemcpyAsync(c->ts, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
   EK_HIP(c, hipMemsetAsync(c->spread, 0, ek::SPREAD_ELEMS * sizeof(unsigned long long), c->stream));
-  EK_HIP(c, launch_inv_dt_into(c, &c->ts[0].invdt_bits));
+  EK_HIP(c, launch_inv_dt_into(c, &c->ts[0].invdt_bits, c->spread)); // slot 0's words
   const int nfull = nsteps / 6, rem = nsteps % 6;
   if (nfull > 0 && (!c->graph || c->graph_nStepmax != nStepmax || c->graph_tEnd != tEnd ||
                     c->graph_U0 != c->U[0]))

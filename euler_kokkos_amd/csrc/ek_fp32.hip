@@ -10,9 +10,9 @@ hipError_t step(int dim, int rs, int st, const float* Uin, float* Uout, const Ge
   return launch_step<float>(dim, rs, st, Uin, Uout, geo, gas, ctl, cz, s);
 }
 hipError_t inv_dt(int dim, const float* U, const Geom<float>& geo, const Gas<float>& gas,
-                  unsigned long long* dst, hipStream_t s)
+                  unsigned long long* dst, unsigned long long* spread, hipStream_t s)
 {
-  return launch_inv_dt<float>(dim, U, geo, gas, dst, s);
+  return launch_inv_dt<float>(dim, U, geo, gas, dst, spread, s);
 }
 hipError_t budget(int nv, const float* U, long long ncell, long long vstride, double* partial,
                   int nblocks, hipStream_t s)

@@ -10,9 +10,9 @@ hipError_t step(int dim, int rs, int st, const double* Uin, double* Uout, const 
   return launch_step<double>(dim, rs, st, Uin, Uout, geo, gas, ctl, cz, s);
 }
 hipError_t inv_dt(int dim, const double* U, const Geom<double>& geo, const Gas<double>& gas,
-                  unsigned long long* dst, hipStream_t s)
+                  unsigned long long* dst, unsigned long long* spread, hipStream_t s)
 {
-  return launch_inv_dt<double>(dim, U, geo, gas, dst, s);
+  return launch_inv_dt<double>(dim, U, geo, gas, dst, spread, s);
 }
 hipError_t budget(int nv, const double* U, long long ncell, long long vstride, double* partial,
                   int nblocks, hipStream_t s)

@@ -197,6 +197,14 @@ __device__ __forceinline__ unsigned long long invdt_gather(const StepCtl& c, int
   return wave_umax(v);
 }
 
+// Word of block blockIdx.x in a standalone CFL reduction: dst itself, or, with spread
+// words (spread + k*SPREAD_STRIDE for k = 1..NSPREAD-1, k = 0 being dst), word b % NSPREAD.
+__device__ __forceinline__ unsigned long long* reduce_word(unsigned long long* dst, unsigned long long* spread)
+{
+  const unsigned k = blockIdx.x & (NSPREAD - 1);
+  return (spread && k) ? spread + (long long)k * SPREAD_STRIDE : dst;
+}
+
 // Block max then one atomic per block. NaN (a failed cell) wins: its bit pattern is above
 // every finite double's, so a blown-up state yields dt = NaN instead of being skipped.
 template <int NT>
@@ -478,7 +486,8 @@ __global__ __launch_bounds__(NT) void hydro2d_step(const T* __restrict__ Uin, T*
 // ------------------------------------------------------------------------------------
 template <class T, int DIM, int NT>
 __global__ __launch_bounds__(NT) void inv_dt_kernel(const T* __restrict__ U, Geom<T> geo, Gas<T> gas,
-                                                    unsigned long long* dst, long long ncell)
+                                                    unsigned long long* dst, unsigned long long* spread,
+                                                    long long ncell)
 {
   __shared__ double sred[NT / 64];
   constexpr int NV = DIM == 3 ? 5 : 4;
@@ -491,7 +500,7 @@ __global__ __launch_bounds__(NT) void inv_dt_kernel(const T* __restrict__ U, Geo
     const double s = cfl_term<T, DIM>(gas, u, geo);
     local = (s > local || !(s == s)) ? s : local;
   }
-  block_max_to<NT>(dst, local, sred);
+  block_max_to<NT>(reduce_word(dst, spread), local, sred);
 }
 
 // Compensated (Neumaier) FP64 sums of each conserved variable over the interior:

@@ -18,7 +18,7 @@ namespace fp64 {
 hipError_t step(int dim, int rs, int st, const double* Uin, double* Uout, const Geom<double>& geo,
                 const Gas<double>& gas, const StepCtl& ctl, int cz, hipStream_t s);
 hipError_t inv_dt(int dim, const double* U, const Geom<double>& geo, const Gas<double>& gas,
-                  unsigned long long* dst, hipStream_t s);
+                  unsigned long long* dst, unsigned long long* spread, hipStream_t s);
 hipError_t budget(int nv, const double* U, long long ncell, long long vstride, double* partial,
                   int nblocks, hipStream_t s);
 hipError_t ref_to_soa(int dim, const double* R, double* U, const Geom<double>& geo, int gw, int nbvar,
@@ -29,7 +29,7 @@ hipError_t soa_to_ref(int dim, const double* U, double* R, const Geom<double>& g
 hipError_t mhd_step(int rs, int st, const double* Uin, double* Uout, const Geom<double>& geo, const Gas<double>& gas,
                     const StepCtl& ctl, hipStream_t s);
 hipError_t mhd_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
-                      hipStream_t s);
+                      unsigned long long* spread, hipStream_t s);
 // job pool (ek_pool_fp64.hip)
 hipError_t pool_step(int dim, int variant, int st, const double* Ua, double* Ub, const PoolJob<double>* jobs,
                      const PoolWork* work, int nwork, TimeSlot* ts, int slot, int grid, hipStream_t s);
@@ -39,14 +39,14 @@ int pool_occupancy(int dim, int variant, int st);
 hipError_t mhd3_step(int rs, int st, const double* Uin, double* Uout, double* scratch, const Geom<double>& geo,
                      const Gas<double>& gas, const StepCtl& ctl, hipStream_t s);
 hipError_t mhd3_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
-                       hipStream_t s);
+                       unsigned long long* spread, hipStream_t s);
 long long mhd3_scratch_elems(int nx, int ny, int nz);
 } // namespace fp64
 namespace fp32 {
 hipError_t step(int dim, int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo,
                 const Gas<float>& gas, const StepCtl& ctl, int cz, hipStream_t s);
 hipError_t inv_dt(int dim, const float* U, const Geom<float>& geo, const Gas<float>& gas,
-                  unsigned long long* dst, hipStream_t s);
+                  unsigned long long* dst, unsigned long long* spread, hipStream_t s);
 hipError_t budget(int nv, const float* U, long long ncell, long long vstride, double* partial,
                   int nblocks, hipStream_t s);
 hipError_t ref_to_soa(int dim, const double* R, float* U, const Geom<float>& geo, int gw, int nbvar,
@@ -57,7 +57,7 @@ hipError_t soa_to_ref(int dim, const float* U, double* R, const Geom<float>& geo
 hipError_t mhd_step(int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo, const Gas<float>& gas,
                     const StepCtl& ctl, hipStream_t s);
 hipError_t mhd_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
-                      hipStream_t s);
+                      unsigned long long* spread, hipStream_t s);
 // job pool (ek_pool_fp32.hip)
 hipError_t pool_step(int dim, int variant, int st, const float* Ua, float* Ub, const PoolJob<float>* jobs,
                      const PoolWork* work, int nwork, TimeSlot* ts, int slot, int grid, hipStream_t s);
@@ -67,7 +67,7 @@ int pool_occupancy(int dim, int variant, int st);
 hipError_t mhd3_step(int rs, int st, const float* Uin, float* Uout, float* scratch, const Geom<float>& geo,
                      const Gas<float>& gas, const StepCtl& ctl, hipStream_t s);
 hipError_t mhd3_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
-                       hipStream_t s);
+                       unsigned long long* spread, hipStream_t s);
 long long mhd3_scratch_elems(int nx, int ny, int nz);
 } // namespace fp32
 } // namespace ek

@@ -68,16 +68,16 @@ inline hipError_t launch_step(int dim, int rs, int st, const T* Uin, T* Uout, co
 
 template <class T>
 inline hipError_t launch_inv_dt(int dim, const T* U, const Geom<T>& geo, const Gas<T>& gas,
-                                unsigned long long* dst, hipStream_t s)
+                                unsigned long long* dst, unsigned long long* spread, hipStream_t s)
 {
   const long long n = geo.vstride;
   long long nb = (n + 255) / 256;
   if (nb > 4096)
     nb = 4096;
   if (dim == 2)
-    hipLaunchKernelGGL((inv_dt_kernel<T, 2, 256>), dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst, n);
+    hipLaunchKernelGGL((inv_dt_kernel<T, 2, 256>), dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst, spread, n);
   else
-    hipLaunchKernelGGL((inv_dt_kernel<T, 3, 256>), dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst, n);
+    hipLaunchKernelGGL((inv_dt_kernel<T, 3, 256>), dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst, spread, n);
   return hipGetLastError();
 }
 

@@ -876,7 +876,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 // the boundary map when they are ghosts.
 template <class T, int NT>
 __global__ __launch_bounds__(NT) void mhd2d_inv_dt_kernel(const T* __restrict__ U, Geom<T> geo, Gas<T> gas,
-                                                           unsigned long long* dst)
+                                                           unsigned long long* dst, unsigned long long* spread)
 {
   __shared__ double sred[NT / 64];
   const long long ncell = geo.vstride;
@@ -901,7 +901,7 @@ __global__ __launch_bounds__(NT) void mhd2d_inv_dt_kernel(const T* __restrict__ 
     const double s = mhd_cfl_term<T>(gas, u, mx, my, geo);
     local = (s > local || !(s == s)) ? s : local;
   }
-  block_max_to<NT>(dst, local, sred);
+  block_max_to<NT>(reduce_word(dst, spread), local, sred);
 }
 
 } // namespace mhd

@@ -7,10 +7,12 @@
 //   prims   : primitives with the right/top/front face fields (ConvertToPrimitivesFunctor3D_MHD)
 //   efield  : edge electric fields and transverse face-field slopes (ComputeElecFieldFunctor3D,
 //             ComputeMagSlopesFunctor3D)
-//   trace   : the 6 face states and 12 edge states of every cell (trace_unsplit_mhd_3d_simpler);
-//             edge states keep the 7 components the edge solver reads, in its own order
-//   fluxemf : 3 Riemann problems and 3 edge EMFs per cell (ComputeFluxesAndStoreFunctor3D_MHD,
-//             ComputeEmfAndStoreFunctor3D)
+//   trace   : the half-step record of every cell (trace_unsplit_mhd_3d_simpler): centred
+//             state, face fields and half slopes, 35 values, from which each of the 6 face
+//             states and 12 edge states is the centre plus one or two half slopes
+//   fluxemf : rebuilds the face and edge states from the records with the reference's own
+//             additions and floors, then 3 Riemann problems and 3 edge EMFs per cell
+//             (ComputeFluxesAndStoreFunctor3D_MHD, ComputeEmfAndStoreFunctor3D)
 //   update  : flux update, CT update and the CFL term of the new state (UpdateFunctor3D_MHD,
 //             UpdateEmfFunctor3D, ComputeDtFunctor3D_MHD), with the device time bookkeeping
 // Every kernel reads the step's dt from the device time slot.
@@ -36,12 +38,17 @@ enum {
   A_Q = 8,            // primitives, 8
   A_E = 16,           // Ex, Ey, Ez
   A_DB = 19,          // dA/dy, dA/dz, dB/dx, dB/dz, dC/dx, dC/dy (full, not halved)
-  A_FACE = 25,        // qm_x, qm_y, qm_z, qp_x, qp_y, qp_z: 6 x 8
-  A_EDGE = 73,        // (RT, RB, LT, LB) x (X, Y, Z): 12 x 7, order r p u v a b c of the edge solver
-  A_FLUX = 157,       // Fx, Fy, Fz: 3 x 5 (ID IP IU IV IW of the update, unrotated)
-  A_EMF = 172,        // EMF x, y, z
-  A_TOTAL = 175
+  A_REC = 25,         // half-step record, K_N values (below)
+  A_FLUX = 60,        // Fx, Fy, Fz: 3 x 5 (ID IP IU IV IW of the update, unrotated)
+  A_EMF = 75,         // EMF x, y, z
+  A_TOTAL = 78
 };
+
+// Half-step record of a cell: K_C + v the centred state (component order of the state),
+// K_AL..K_CR the six face fields, then the half slopes along x, y and z, each group
+// holding ID IP IU IV IW at 0..4 and the two transverse field components at 5, 6
+// (x: IB IC; y: IA IC; z: IA IB).
+enum { K_C = 0, K_AL = 8, K_AR, K_BL, K_BR, K_CL, K_CR, K_X = 14, K_Y = 21, K_Z = 28, K_N = 35 };
 
 __device__ __forceinline__ long long pidx(const Work3& w, int i, int j, int k)
 {
@@ -242,115 +249,191 @@ __global__ __launch_bounds__(256) void trace_kernel(T* __restrict__ S, Work3 w, 
   BR = BR + sBR0;
   CL = CL + sCL0;
   CR = CR + sCR0;
-  const T smallR = gas.smallr, smallp = gas.smallp;
-  // face state f (0..5 = qm_x, qm_y, qm_z, qp_x, qp_y, qp_z); 3D floors p by smallp alone
-  auto face = [&](int f, T d0, T u0, T v0, T w0, T p0, T a0, T b0, T c0) {
-    T* F = S + (long long)(A_FACE + 8 * f) * PN + o;
-    d0 = emax(smallR, d0);
-    p0 = emax(smallp, p0);
-    F[ID * PN] = d0;
-    F[IU * PN] = u0;
-    F[IV * PN] = v0;
-    F[IW * PN] = w0;
-    F[IP * PN] = p0;
-    F[IA * PN] = a0;
-    F[IB * PN] = b0;
-    F[IC * PN] = c0;
-  };
-  face(3, r - drx, u - dux, v - dvx, ww - dwx, p - dpx, AL, B - dBx, C - dCx);
-  face(0, r + drx, u + dux, v + dvx, ww + dwx, p + dpx, AR, B + dBx, C + dCx);
-  face(4, r - dry, u - duy, v - dvy, ww - dwy, p - dpy, A - dAy, BL, C - dCy);
-  face(1, r + dry, u + duy, v + dvy, ww + dwy, p + dpy, A + dAy, BR, C + dCy);
-  face(5, r - drz, u - duz, v - dvz, ww - dwz, p - dpz, A - dAz, B - dBz, CL);
-  face(2, r + drz, u + duz, v + dvz, ww + dwz, p + dpz, A + dAz, B + dBz, CR);
-  // edge state e (0..3 = RT, RB, LT, LB) in direction d (0..2 = X, Y, Z), stored as the
-  // edge solver's (r, p, u, v, a, b, c): X -> (IV, IW, IB, IC, IA), Y -> (IW, IU, IC, IA, IB),
-  // Z -> (IU, IV, IA, IB, IC)
-  auto edge = [&](int e, int d, T d0, T u0, T v0, T w0, T p0, T a0, T b0, T c0) {
-    T* G = S + (long long)(A_EDGE + 7 * (3 * e + d)) * PN + o;
-    d0 = emax(smallR, d0);
-    p0 = emax(smallp, p0);
-    G[0] = d0;
-    G[PN] = p0;
-    if (d == 0) {
-      G[2 * PN] = v0; G[3 * PN] = w0; G[4 * PN] = b0; G[5 * PN] = c0; G[6 * PN] = a0;
-    } else if (d == 1) {
-      G[2 * PN] = w0; G[3 * PN] = u0; G[4 * PN] = c0; G[5 * PN] = a0; G[6 * PN] = b0;
-    } else {
-      G[2 * PN] = u0; G[3 * PN] = v0; G[4 * PN] = a0; G[5 * PN] = b0; G[6 * PN] = c0;
-    }
-  };
-  edge(0, 0, r + (+dry + drz), u + (+duy + duz), v + (+dvy + dvz), ww + (+dwy + dwz), p + (+dpy + dpz),
-       A + (+dAy + dAz), BR + (+dBRz), CR + (+dCRy));
-  edge(1, 0, r + (+dry - drz), u + (+duy - duz), v + (+dvy - dvz), ww + (+dwy - dwz), p + (+dpy - dpz),
-       A + (+dAy - dAz), BR + (-dBRz), CL + (+dCLy));
-  edge(2, 0, r + (-dry + drz), u + (-duy + duz), v + (-dvy + dvz), ww + (-dwy + dwz), p + (-dpy + dpz),
-       A + (-dAy + dAz), BL + (+dBLz), CR + (-dCRy));
-  edge(3, 0, r + (-dry - drz), u + (-duy - duz), v + (-dvy - dvz), ww + (-dwy - dwz), p + (-dpy - dpz),
-       A + (-dAy - dAz), BL + (-dBLz), CL + (-dCLy));
-  edge(0, 1, r + (+drx + drz), u + (+dux + duz), v + (+dvx + dvz), ww + (+dwx + dwz), p + (+dpx + dpz),
-       AR + (+dARz), B + (+dBx + dBz), CR + (+dCRx));
-  edge(1, 1, r + (+drx - drz), u + (+dux - duz), v + (+dvx - dvz), ww + (+dwx - dwz), p + (+dpx - dpz),
-       AR + (-dARz), B + (+dBx - dBz), CL + (+dCLx));
-  edge(2, 1, r + (-drx + drz), u + (-dux + duz), v + (-dvx + dvz), ww + (-dwx + dwz), p + (-dpx + dpz),
-       AL + (+dALz), B + (-dBx + dBz), CR + (-dCRx));
-  edge(3, 1, r + (-drx - drz), u + (-dux - duz), v + (-dvx - dvz), ww + (-dwx - dwz), p + (-dpx - dpz),
-       AL + (-dALz), B + (-dBx - dBz), CL + (-dCLx));
-  edge(0, 2, r + (+drx + dry), u + (+dux + duy), v + (+dvx + dvy), ww + (+dwx + dwy), p + (+dpx + dpy),
-       AR + (+dARy), BR + (+dBRx), C + (+dCx + dCy));
-  edge(1, 2, r + (+drx - dry), u + (+dux - duy), v + (+dvx - dvy), ww + (+dwx - dwy), p + (+dpx - dpy),
-       AR + (-dARy), BL + (+dBLx), C + (+dCx - dCy));
-  edge(2, 2, r + (-drx + dry), u + (-dux + duy), v + (-dvx + dvy), ww + (-dwx + dwy), p + (-dpx + dpy),
-       AL + (+dALy), BR + (-dBRx), C + (-dCx + dCy));
-  edge(3, 2, r + (-drx - dry), u + (-dux - duy), v + (-dvx - dvy), ww + (-dwx - dwy), p + (-dpx - dpy),
-       AL + (-dALy), BL + (-dBLx), C + (-dCx - dCy));
+  T* Rw = S + (long long)A_REC * PN + o;
+  auto put = [&](int kk, T x) { Rw[(long long)kk * PN] = x; };
+  put(K_C + ID, r);
+  put(K_C + IP, p);
+  put(K_C + IU, u);
+  put(K_C + IV, v);
+  put(K_C + IW, ww);
+  put(K_C + IA, A);
+  put(K_C + IB, B);
+  put(K_C + IC, C);
+  put(K_AL, AL);
+  put(K_AR, AR);
+  put(K_BL, BL);
+  put(K_BR, BR);
+  put(K_CL, CL);
+  put(K_CR, CR);
+  put(K_X + 0, drx);
+  put(K_X + 1, dpx);
+  put(K_X + 2, dux);
+  put(K_X + 3, dvx);
+  put(K_X + 4, dwx);
+  put(K_X + 5, dBx);
+  put(K_X + 6, dCx);
+  put(K_Y + 0, dry);
+  put(K_Y + 1, dpy);
+  put(K_Y + 2, duy);
+  put(K_Y + 3, dvy);
+  put(K_Y + 4, dwy);
+  put(K_Y + 5, dAy);
+  put(K_Y + 6, dCy);
+  put(K_Z + 0, drz);
+  put(K_Z + 1, dpz);
+  put(K_Z + 2, duz);
+  put(K_Z + 3, dvz);
+  put(K_Z + 4, dwz);
+  put(K_Z + 5, dAz);
+  put(K_Z + 6, dBz);
+}
+
+// Element at a 32-bit byte offset from a wave-uniform array base: the record and slope
+// reads below then address as scalar base + one 32-bit lane offset per cell, instead of a
+// 64-bit address per (array, cell), which would need two registers each. The work area
+// of one grid is below 4 GiB per array (checked by the host).
+template <class T>
+__device__ __forceinline__ T ldb(const T* base, unsigned boff)
+{
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
 }
 
 template <class T>
-__device__ __forceinline__ Edge<T> load_edge(const T* S, long long PN, int e, int d, long long o)
+struct RecView {
+  const T* S;
+  long long PN;
+  __device__ __forceinline__ T operator()(int kk, unsigned boff) const
+  {
+    return ldb(S + (long long)(A_REC + kk) * PN, boff);
+  }
+  __device__ __forceinline__ T db(int m, unsigned boff) const
+  {
+    return ldb(S + (long long)(A_DB + m) * PN, boff);
+  }
+};
+
+// Face state of the cell at byte offset cb along dir (0..2): maxface -> qm (centre + half
+// slope, FACE_*MAX), else qp (centre - half slope); the normal field is the face's own.
+// Floors as the 3D trace (trace_unsplit_mhd_3d_simpler): rho by smallr, p by smallp.
+template <class T>
+__device__ __forceinline__ void face_state(const RecView<T>& R, unsigned cb, int dir, bool maxface,
+                                           const Gas<T>& gas, T* q)
 {
-  const T* G = S + (long long)(A_EDGE + 7 * (3 * e + d)) * PN + o;
+  const int K = K_X + 7 * dir;
+#pragma unroll
+  for (int v = 0; v < 5; v++)
+    q[v] = maxface ? R(K_C + v, cb) + R(K + v, cb) : R(K_C + v, cb) - R(K + v, cb);
+  const int m1 = dir == 0 ? IB : IA, m2 = dir == 2 ? IB : IC, mn = dir == 0 ? IA : (dir == 1 ? IB : IC);
+  q[m1] = maxface ? R(K_C + m1, cb) + R(K + 5, cb) : R(K_C + m1, cb) - R(K + 5, cb);
+  q[m2] = maxface ? R(K_C + m2, cb) + R(K + 6, cb) : R(K_C + m2, cb) - R(K + 6, cb);
+  q[mn] = R((maxface ? K_AR : K_AL) + 2 * dir, cb);
+  q[ID] = emax(gas.smallr, q[ID]);
+  q[IP] = emax(gas.smallp, q[IP]);
+}
+
+// Edge state e (0..3 = RT, RB, LT, LB) along d (0..2 = X, Y, Z) of the cell at byte offset
+// cb, as the edge solver reads it. Over the two transverse directions (X: y, z; Y: x, z;
+// Z: x, y) a state is centre + (+-s1 +- s2) (RT ++, RB +-, LT -+, LB --); the two
+// transverse face fields are the right or left face value plus or minus its own
+// transverse half slope, h*DB at that face (ComputeMagSlopesFunctor3D's slopes, halved as
+// the trace does). bx, by, bz: byte strides of the padded grid.
+template <class T>
+__device__ __forceinline__ Edge<T> edge_state(const RecView<T>& R, unsigned cb, int e, int d, unsigned bx,
+                                              unsigned by, unsigned bz, const Gas<T>& gas)
+{
+  const T h = T(0.5);
+  const int K1 = d == 0 ? K_Y : K_X, K2 = d == 2 ? K_Y : K_Z;
+  const bool p1 = e < 2, p2 = (e & 1) == 0;
+  auto comb = [&](T base, T a, T b) {
+    const T t = p1 ? a : -a;
+    return base + (p2 ? t + b : t - b);
+  };
+  T q[NV];
+#pragma unroll
+  for (int v = 0; v < 5; v++)
+    q[v] = comb(R(K_C + v, cb), R(K1 + v, cb), R(K2 + v, cb));
+  if (d == 0) {
+    q[IA] = comb(R(K_C + IA, cb), R(K_Y + 5, cb), R(K_Z + 5, cb));
+    const T sb = p1 ? h * R.db(3, cb + by) : h * R.db(3, cb);   // dBRz / dBLz
+    q[IB] = (p1 ? R(K_BR, cb) : R(K_BL, cb)) + (p2 ? sb : -sb);
+    const T sc = p2 ? h * R.db(5, cb + bz) : h * R.db(5, cb);   // dCRy / dCLy
+    q[IC] = (p2 ? R(K_CR, cb) : R(K_CL, cb)) + (p1 ? sc : -sc);
+  } else if (d == 1) {
+    q[IB] = comb(R(K_C + IB, cb), R(K_X + 5, cb), R(K_Z + 6, cb));
+    const T sa = p1 ? h * R.db(1, cb + bx) : h * R.db(1, cb);   // dARz / dALz
+    q[IA] = (p1 ? R(K_AR, cb) : R(K_AL, cb)) + (p2 ? sa : -sa);
+    const T sc = p2 ? h * R.db(4, cb + bz) : h * R.db(4, cb);   // dCRx / dCLx
+    q[IC] = (p2 ? R(K_CR, cb) : R(K_CL, cb)) + (p1 ? sc : -sc);
+  } else {
+    q[IC] = comb(R(K_C + IC, cb), R(K_X + 6, cb), R(K_Y + 6, cb));
+    const T sa = p1 ? h * R.db(0, cb + bx) : h * R.db(0, cb);   // dARy / dALy
+    q[IA] = (p1 ? R(K_AR, cb) : R(K_AL, cb)) + (p2 ? sa : -sa);
+    const T sb = p2 ? h * R.db(2, cb + by) : h * R.db(2, cb);   // dBRx / dBLx
+    q[IB] = (p2 ? R(K_BR, cb) : R(K_BL, cb)) + (p1 ? sb : -sb);
+  }
   Edge<T> s;
-  s.r = G[0];
-  s.p = G[PN];
-  s.u = G[2 * PN];
-  s.v = G[3 * PN];
-  s.a = G[4 * PN];
-  s.b = G[5 * PN];
-  s.c = G[6 * PN];
+  s.r = emax(gas.smallr, q[ID]);
+  s.p = emax(gas.smallp, q[IP]);
+  if (d == 0) {
+    s.u = q[IV]; s.v = q[IW]; s.a = q[IB]; s.b = q[IC]; s.c = q[IA];
+  } else if (d == 1) {
+    s.u = q[IW]; s.v = q[IU]; s.a = q[IC]; s.b = q[IA]; s.c = q[IB];
+  } else {
+    s.u = q[IU]; s.v = q[IV]; s.a = q[IA]; s.b = q[IB]; s.c = q[IC];
+  }
   return s;
 }
 
-// fluxes and EMFs on 3 <= i,j,k < size-2
+// Fluxes and EMFs on 3 <= i,j,k < size-2. Four tasks per chunk of 256 cells: the three
+// Riemann problems of each cell, or one of the three edge EMFs. One thread doing all six
+// holds too many rebuilt states at once (over 200 VGPRs); one task each stays near 128.
+// Launch 32 * ceil(nchunk / 8) blocks: block b takes task (b / 8) % 4 of chunk
+// (b / 32) * 8 + b % 8, so the four tasks of a chunk run on one XCD (blocks are dealt to
+// the XCDs round robin) and share its L2 copy of the chunk's records.
 template <class T, int RS>
 __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w, Gas<T> gas, StepCtl ctl)
 {
   double dt;
   if (!step_dt(ctl, dt))
     return;
-  int i, j, k;
-  if (!cell_of<T>(w, i, j, k) || i < 3 || j < 3 || k < 3 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
+  const unsigned b = blockIdx.x;
+  const int task = (b >> 3) & 3;
+  const long long c = ((long long)(((b >> 5) << 3) | (b & 7))) * 256 + threadIdx.x;
+  if (c >= w.PN)
     return;
-  const long long PN = w.PN, o = pidx(w, i, j, k);
-  const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
+  const int i = (int)(c % w.PX), j = (int)((c / w.PX) % w.PY), k = (int)(c / ((long long)w.PX * w.PY));
+  if (i < 3 || j < 3 || k < 3 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
+    return;
+  const long long PN = w.PN, o = c;
+  const unsigned ob = (unsigned)o * (unsigned)sizeof(T), bx = sizeof(T), by = (unsigned)w.PX * (unsigned)sizeof(T);
+  const unsigned bz = (unsigned)(w.PX * w.PY) * (unsigned)sizeof(T);
+  const RecView<T> R{ S, PN };
+  if (task != 0) {
+    auto E = [&](unsigned cb, int e, int d) { return edge_state<T>(R, cb, e, d, bx, by, bz, gas); };
+    if (task == 1) {
+      // EMF z: RT(i-1,j-1,k), RB(i-1,j,k), LT(i,j-1,k), LB(i,j,k) of the Z edges
+      S[(A_EMF + 2) * PN + o] = emf_z<T>(gas, E(ob - bx - by, 0, 2), E(ob - bx, 1, 2), E(ob - by, 2, 2), E(ob, 3, 2));
+    } else if (task == 2) {
+      // EMF y: RT(i-1,j,k-1), LT(i,j,k-1) in the RB place, RB(i-1,j,k) in the LT place, LB(i,j,k)
+      S[(A_EMF + 1) * PN + o] = emf_z<T>(gas, E(ob - bx - bz, 0, 1), E(ob - bz, 2, 1), E(ob - bx, 1, 1), E(ob, 3, 1));
+    } else {
+      // EMF x: RT(i,j-1,k-1), RB(i,j-1,k), LT(i,j,k-1), LB(i,j,k)
+      S[(A_EMF + 0) * PN + o] = emf_z<T>(gas, E(ob - by - bz, 0, 0), E(ob - by, 1, 0), E(ob - bz, 2, 0), E(ob, 3, 0));
+    }
+    return;
+  }
   T ql[NV], qr[NV], f[NV];
   // x: qm_x(i-1) | qp_x(i)
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    ql[v] = S[(A_FACE + 0 + v) * PN + o - sx];
-    qr[v] = S[(A_FACE + 24 + v) * PN + o];
-  }
+  face_state<T>(R, ob - bx, 0, true, gas, ql);
+  face_state<T>(R, ob, 0, false, gas, qr);
   riemann<T, RS>(gas, ql, qr, f);
   {
     T* F = S + (A_FLUX + 0) * PN + o;
     F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IU]; F[3 * PN] = f[IV]; F[4 * PN] = f[IW];
   }
   // y: qm_y(j-1) | qp_y(j), IU<->IV and IA<->IB around the solve
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    ql[v] = S[(A_FACE + 8 + v) * PN + o - sy];
-    qr[v] = S[(A_FACE + 32 + v) * PN + o];
-  }
+  face_state<T>(R, ob - by, 1, true, gas, ql);
+  face_state<T>(R, ob, 1, false, gas, qr);
   {
     T t;
     t = ql[IU]; ql[IU] = ql[IV]; ql[IV] = t;
@@ -364,11 +447,8 @@ __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w
     F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IV]; F[3 * PN] = f[IU]; F[4 * PN] = f[IW];
   }
   // z: qm_z(k-1) | qp_z(k), IU<->IW and IA<->IC around the solve
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    ql[v] = S[(A_FACE + 16 + v) * PN + o - sz];
-    qr[v] = S[(A_FACE + 40 + v) * PN + o];
-  }
+  face_state<T>(R, ob - bz, 2, true, gas, ql);
+  face_state<T>(R, ob, 2, false, gas, qr);
   {
     T t;
     t = ql[IU]; ql[IU] = ql[IW]; ql[IW] = t;
@@ -381,15 +461,6 @@ __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w
     T* F = S + (A_FLUX + 10) * PN + o;
     F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IW]; F[3 * PN] = f[IV]; F[4 * PN] = f[IU];
   }
-  // EMF z: RT(i-1,j-1,k), RB(i-1,j,k), LT(i,j-1,k), LB(i,j,k) of the Z edges
-  S[(A_EMF + 2) * PN + o] = emf_z<T>(gas, load_edge(S, PN, 0, 2, o - sx - sy), load_edge(S, PN, 1, 2, o - sx),
-                                     load_edge(S, PN, 2, 2, o - sy), load_edge(S, PN, 3, 2, o));
-  // EMF y: RT(i-1,j,k-1), LT(i,j,k-1) in the RB place, RB(i-1,j,k) in the LT place, LB(i,j,k)
-  S[(A_EMF + 1) * PN + o] = emf_z<T>(gas, load_edge(S, PN, 0, 1, o - sx - sz), load_edge(S, PN, 2, 1, o - sz),
-                                     load_edge(S, PN, 1, 1, o - sx), load_edge(S, PN, 3, 1, o));
-  // EMF x: RT(i,j-1,k-1), RB(i,j-1,k), LT(i,j,k-1), LB(i,j,k)
-  S[(A_EMF + 0) * PN + o] = emf_z<T>(gas, load_edge(S, PN, 0, 0, o - sy - sz), load_edge(S, PN, 1, 0, o - sy),
-                                     load_edge(S, PN, 2, 0, o - sz), load_edge(S, PN, 3, 0, o));
 }
 
 // CFL term of a 3D cell from its new state and new right/top/front face fields
@@ -521,7 +592,7 @@ __global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T*
 // boundary map
 template <class T>
 __global__ __launch_bounds__(256) void inv_dt3_kernel(const T* __restrict__ U, Geom<T> geo, Gas<T> gas,
-                                                      unsigned long long* dst)
+                                                      unsigned long long* dst, unsigned long long* spread)
 {
   __shared__ double sred[4];
   const long long n = geo.vstride;
@@ -549,7 +620,7 @@ __global__ __launch_bounds__(256) void inv_dt3_kernel(const T* __restrict__ U, G
     const double s = cfl3<T>(gas, u, mx, my, mz, geo);
     local = (s > local || !(s == s)) ? s : local;
   }
-  block_max_to<256>(dst, local, sred);
+  block_max_to<256>(reduce_word(dst, spread), local, sred);
 }
 
 } // namespace mhd3

@@ -16,7 +16,8 @@ static hipError_t launch3(const float* Uin, float* Uout, float* S, const mhd3::W
   hipLaunchKernelGGL(mhd3::prims_kernel<float>, dim3(nb), dim3(256), 0, s, S, w, gas, ctl);
   hipLaunchKernelGGL((mhd3::efield_kernel<float, ST>), dim3(nb), dim3(256), 0, s, S, w, ctl);
   hipLaunchKernelGGL((mhd3::trace_kernel<float, ST>), dim3(nb), dim3(256), 0, s, S, w, geo, gas, ctl);
-  hipLaunchKernelGGL((mhd3::fluxemf_kernel<float, RS>), dim3(nb), dim3(256), 0, s, S, w, gas, ctl);
+  const unsigned nfe = 32u * ((nb + 7u) / 8u); // 4 tasks per chunk, chunks in groups of 8
+  hipLaunchKernelGGL((mhd3::fluxemf_kernel<float, RS>), dim3(nfe), dim3(256), 0, s, S, w, gas, ctl);
   hipLaunchKernelGGL(mhd3::update_kernel<float>, dim3(ni), dim3(256), 0, s, S, Uout, w, geo, gas, ctl);
   return hipGetLastError();
 }
@@ -44,12 +45,12 @@ hipError_t mhd3_step(int rs, int st, const float* Uin, float* Uout, float* scrat
 }
 
 hipError_t mhd3_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
-                       hipStream_t s)
+                       unsigned long long* spread, hipStream_t s)
 {
   long long nb = (geo.vstride + 255) / 256;
   if (nb > 4096)
     nb = 4096;
-  hipLaunchKernelGGL(mhd3::inv_dt3_kernel<float>, dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst);
+  hipLaunchKernelGGL(mhd3::inv_dt3_kernel<float>, dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst, spread);
   return hipGetLastError();
 }
 

@@ -16,7 +16,8 @@ static hipError_t launch3(const double* Uin, double* Uout, double* S, const mhd3
   hipLaunchKernelGGL(mhd3::prims_kernel<double>, dim3(nb), dim3(256), 0, s, S, w, gas, ctl);
   hipLaunchKernelGGL((mhd3::efield_kernel<double, ST>), dim3(nb), dim3(256), 0, s, S, w, ctl);
   hipLaunchKernelGGL((mhd3::trace_kernel<double, ST>), dim3(nb), dim3(256), 0, s, S, w, geo, gas, ctl);
-  hipLaunchKernelGGL((mhd3::fluxemf_kernel<double, RS>), dim3(nb), dim3(256), 0, s, S, w, gas, ctl);
+  const unsigned nfe = 32u * ((nb + 7u) / 8u); // 4 tasks per chunk, chunks in groups of 8
+  hipLaunchKernelGGL((mhd3::fluxemf_kernel<double, RS>), dim3(nfe), dim3(256), 0, s, S, w, gas, ctl);
   hipLaunchKernelGGL(mhd3::update_kernel<double>, dim3(ni), dim3(256), 0, s, S, Uout, w, geo, gas, ctl);
   return hipGetLastError();
 }
@@ -44,12 +45,12 @@ hipError_t mhd3_step(int rs, int st, const double* Uin, double* Uout, double* sc
 }
 
 hipError_t mhd3_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
-                       hipStream_t s)
+                       unsigned long long* spread, hipStream_t s)
 {
   long long nb = (geo.vstride + 255) / 256;
   if (nb > 4096)
     nb = 4096;
-  hipLaunchKernelGGL(mhd3::inv_dt3_kernel<double>, dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst);
+  hipLaunchKernelGGL(mhd3::inv_dt3_kernel<double>, dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst, spread);
   return hipGetLastError();
 }
 

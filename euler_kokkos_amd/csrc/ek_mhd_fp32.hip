@@ -10,9 +10,9 @@ hipError_t mhd_step(int rs, int st, const float* Uin, float* Uout, const Geom<fl
   return mhd::launch_step<float>(rs, st, Uin, Uout, geo, gas, ctl, s);
 }
 hipError_t mhd_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
-                      hipStream_t s)
+                      unsigned long long* spread, hipStream_t s)
 {
-  return mhd::launch_inv_dt<float>(U, geo, gas, dst, s);
+  return mhd::launch_inv_dt<float>(U, geo, gas, dst, spread, s);
 }
 } // namespace fp32
 } // namespace ek

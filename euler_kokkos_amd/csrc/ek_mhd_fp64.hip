@@ -10,9 +10,9 @@ hipError_t mhd_step(int rs, int st, const double* Uin, double* Uout, const Geom<
   return mhd::launch_step<double>(rs, st, Uin, Uout, geo, gas, ctl, s);
 }
 hipError_t mhd_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
-                      hipStream_t s)
+                      unsigned long long* spread, hipStream_t s)
 {
-  return mhd::launch_inv_dt<double>(U, geo, gas, dst, s);
+  return mhd::launch_inv_dt<double>(U, geo, gas, dst, spread, s);
 }
 } // namespace fp64
 } // namespace ek

@@ -38,12 +38,12 @@ inline hipError_t launch_step(int rs, int st, const T* Uin, T* Uout, const Geom<
 
 template <class T>
 inline hipError_t launch_inv_dt(const T* U, const Geom<T>& geo, const Gas<T>& gas, unsigned long long* dst,
-                                hipStream_t s)
+                                unsigned long long* spread, hipStream_t s)
 {
   long long nb = (geo.vstride + 255) / 256;
   if (nb > 4096)
     nb = 4096;
-  hipLaunchKernelGGL((mhd2d_inv_dt_kernel<T, 256>), dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst);
+  hipLaunchKernelGGL((mhd2d_inv_dt_kernel<T, 256>), dim3((unsigned)nb), dim3(256), 0, s, U, geo, gas, dst, spread);
   return hipGetLastError();
 }
 

@@ -477,12 +477,12 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
     ek::Geom<double> g;
     ek::Gas<double> gs;
     ek::fill_geom<double>(*jp, g, gs);
-    PHIP(p, ek::fp64::inv_dt(jp->dim, (const double*)dst, g, gs, bits, p->stream));
+    PHIP(p, ek::fp64::inv_dt(jp->dim, (const double*)dst, g, gs, bits, nullptr, p->stream));
   } else {
     ek::Geom<float> g;
     ek::Gas<float> gs;
     ek::fill_geom<float>(*jp, g, gs);
-    PHIP(p, ek::fp32::inv_dt(jp->dim, (const float*)dst, g, gs, bits, p->stream));
+    PHIP(p, ek::fp32::inv_dt(jp->dim, (const float*)dst, g, gs, bits, nullptr, p->stream));
   }
   if ((rc = upload_record(p, id)))
     return rc;

@@ -107,7 +107,7 @@ class ContextGroup:
     per-job ghost-padded work area), admitted against a byte budget. Same interface as a
     Pool: admit / step / status / download / retire."""
 
-    WORK_ARRAYS = 175  # reals per padded cell in the 3D MHD work area
+    WORK_ARRAYS = 78  # reals per padded cell in the 3D MHD work area (mhd3::A_TOTAL)
 
     def __init__(self, budget_bytes: int):
         self.budget = int(budget_bytes)
