@@ -1,11 +1,13 @@
 """Serving: a request loop that admits simulation jobs into device-resident job pools by
 their HBM footprint and advances every admitted job together (continuous batching).
 
-Jobs arrive on a seeded Poisson process. At every loop turn the server admits, in arrival
-order, each job that has arrived and fits in its pool's arena; then every pool with work
-advances all of its jobs by one step in one launch; finished jobs are collected and
-retired, which frees memory for the queue. Time-to-solution is measured from a job's
-arrival to its completion; throughput over the whole stream.
+Jobs arrive on a seeded Poisson process. Each job kind has its own pool and its own loop,
+the loops running side by side: at every turn a loop admits, in arrival order, each job of
+its kind that has arrived and fits in the pool's memory; then it advances all admitted jobs
+of the pool together (one launch for a hydro pool, every context's stream at once for 3D
+MHD, whose contexts are created and filled on a background thread); finished jobs are
+collected and retired, which frees memory for the queue. Time-to-solution is measured from
+a job's arrival to its completion; throughput over the whole stream.
 
 A job's result does not depend on what else runs with it: the pool steps each job with
 the per-tile code of its single-grid run (see tests/test_serving_gpu.py).
@@ -15,6 +17,7 @@ from __future__ import annotations
 import math
 import time
 from collections import deque
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -105,15 +108,22 @@ def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh", "mhd_
 class ContextGroup:
     """Jobs that each run in their own device context (3D MHD: the step pipeline needs a
     per-job ghost-padded work area), admitted against a byte budget. Same interface as a
-    Pool: admit / step / status / download / retire."""
+    Pool: admit / step / status / download / retire. Admission reserves the job's bytes at
+    once and creates and uploads its context on a background thread, so the host-to-device
+    copy of a new job overlaps the stepping of the admitted ones; a job steps once its
+    context is ready. A step advances every ready, unfinished job on its own context's
+    stream from a small thread pool (the C calls release the GIL), so small grids overlap on
+    the GPU instead of running one after another."""
 
     WORK_ARRAYS = 78  # reals per padded cell in the 3D MHD work area (mhd3::A_TOTAL)
 
-    def __init__(self, budget_bytes: int):
+    def __init__(self, budget_bytes: int, workers: int = 8):
         self.budget = int(budget_bytes)
         self.used = 0
         self.jobs = {}
         self.next_id = 0
+        self._ex = ThreadPoolExecutor(max_workers=workers)
+        self._up = ThreadPoolExecutor(max_workers=2)
 
     def job_bytes(self, p) -> int:
         elem = 8 if p.precision == native.EK_FP64 else 4
@@ -121,45 +131,80 @@ class ContextGroup:
         padded = (p.nx + 6) * (p.ny + 6) * (p.nz + 6)
         return elem * (2 * 8 * cells + self.WORK_ARRAYS * padded) + (1 << 20)
 
+    @staticmethod
+    def _create(p, U):
+        ctx = native.Context(p)
+        try:
+            ctx.upload(U, native.EK_LAYOUT_SOA)
+            ctx.sync()
+        except Exception:
+            ctx.close()
+            raise
+        return ctx
+
     def admit(self, p, U, nStepmax, tEnd=1e30):
         need = self.job_bytes(p)
         if self.used + need > self.budget:
             raise PoolFull("context group budget exhausted")
-        ctx = native.Context(p)
-        ctx.upload(U, native.EK_LAYOUT_SOA)
         jid = self.next_id
         self.next_id += 1
-        self.jobs[jid] = dict(ctx=ctx, t=0.0, it=0, nStepmax=nStepmax, tEnd=tEnd, need=need)
+        self.jobs[jid] = dict(ctx=None, fut=self._up.submit(self._create, p, U), t=0.0, it=0, nStepmax=nStepmax,
+                              tEnd=tEnd, need=need)
         self.used += need
         return jid
 
+    @staticmethod
+    def _ready(j) -> bool:
+        if j["ctx"] is None and j["fut"].done():
+            j["ctx"] = j["fut"].result()
+        return j["ctx"] is not None
+
     def step(self, nsteps=1):
-        for j in self.jobs.values():
-            if j["it"] < j["nStepmax"] and j["t"] < j["tEnd"] - 1e-14:
-                j["t"], j["it"], _ = j["ctx"].run(nsteps, j["nStepmax"], j["tEnd"], j["t"], j["it"])
+        live = [j for j in self.jobs.values() if j["it"] < j["nStepmax"] and j["t"] < j["tEnd"] - 1e-14]
+        todo = [j for j in live if self._ready(j)]
+        if not todo and live:
+            # nothing uploaded yet: wait for the first context instead of spinning
+            wait([j["fut"] for j in live], return_when=FIRST_COMPLETED)
+            todo = [j for j in live if self._ready(j)]
+
+        def one(j):
+            j["t"], j["it"], _ = j["ctx"].run(nsteps, j["nStepmax"], j["tEnd"], j["t"], j["it"])
+
+        if len(todo) == 1:
+            one(todo[0])
+        elif todo:
+            list(self._ex.map(one, todo))
 
     def status(self, jid):
         j = self.jobs[jid]
         return j["t"], j["it"], j["it"] >= j["nStepmax"] or j["t"] >= j["tEnd"] - 1e-14
 
     def download(self, jid):
-        return self.jobs[jid]["ctx"].download(native.EK_LAYOUT_SOA)
+        j = self.jobs[jid]
+        if j["ctx"] is None:
+            j["ctx"] = j["fut"].result()
+        return j["ctx"].download(native.EK_LAYOUT_SOA)
 
     def retire(self, jid):
         j = self.jobs.pop(jid)
-        j["ctx"].close()
+        ctx = j["ctx"] if j["ctx"] is not None else j["fut"].result()
+        ctx.close()
         self.used -= j["need"]
 
     def close(self):
         for jid in list(self.jobs):
             self.retire(jid)
+        self._ex.shutdown()
+        self._up.shutdown()
 
 
 class Server:
-    """Job pools per kind sharing an HBM budget; continuous batching over arrivals."""
+    """Job pools per kind sharing an HBM budget; continuous batching over arrivals. Each
+    kind's pool has its own loop (admission in arrival order as memory allows, then all of
+    its admitted jobs advanced together), and the loops run side by side."""
 
     def __init__(self, hbm_bytes: int, precision: int = native.EK_FP32, kinds=("hydro_kh", "mhd_blast"),
-                 max_jobs=512):
+                 max_jobs=512, max_chunk=6):
         self.pools = {}
         share = hbm_bytes // len(kinds)
         for k in kinds:
@@ -168,55 +213,65 @@ class Server:
             else:
                 self.pools[k] = Pool(job_params(k, 64, precision), share, max_jobs)
         self.precision = precision
-        self.step_s = 0.0  # wall time of one pool step (all busy pools), last measured
+        self.max_chunk = max_chunk  # most steps advanced per loop turn (between admissions)
+        self._ex = ThreadPoolExecutor(max_workers=len(self.pools))
 
     def close(self):
         for p in self.pools.values():
             p.close()
+        self._ex.shutdown()
 
     def serve(self, jobs, keep_results: bool = False, clock=time.perf_counter, sleep=time.sleep):
-        pending = deque(sorted(jobs, key=lambda j: j.arrival))
-        active = {}  # (kind, jid) -> Job
+        """Serve a job stream: one admission-and-stepping loop per job kind (each kind has its
+        own pool and its own share of HBM), the loops running side by side."""
         t0 = clock()
+        by_kind = {k: [j for j in jobs if j.kind == k] for k in self.pools}
+        futs = [self._ex.submit(self._serve_kind, k, js, t0, keep_results, clock, sleep)
+                for k, js in by_kind.items() if js]
+        launches = sum(f.result() for f in futs)
+        return summarize(jobs, launches)
+
+    def _serve_kind(self, kind, jobs, t0, keep_results, clock, sleep) -> int:
+        pool = self.pools[kind]
+        pending = deque(sorted(jobs, key=lambda j: j.arrival))
+        active = {}  # jid -> Job
         launches = 0
+        step_s = 0.0  # wall time of one step of this pool, last measured
         while pending or active:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 j = pending[0]
                 try:
-                    jid = self.pools[j.kind].admit(j.params, j.U0, j.steps)
+                    jid = pool.admit(j.params, j.U0, j.steps)
                 except PoolFull:
                     break  # first come, first served: wait for memory
                 j.admit_t = clock() - t0
-                active[(j.kind, jid)] = j
+                active[jid] = j
                 pending.popleft()
             if not active:
                 if pending:
                     sleep(max(0.0, pending[0].arrival - (clock() - t0)))
                 continue
-            # advance to the earliest completion (at most a graph's worth of steps), but do
-            # not run past the next arrival by more than one step
-            busy = {k for k, _ in active}
-            nstep = max(1, min(6, min(j.steps - self.pools[k].status(jid)[1] for (k, jid), j in active.items())))
-            if pending and self.step_s > 0.0:
+            # advance to the earliest completion (at most max_chunk steps), but do not run
+            # past the next arrival by more than one step
+            nstep = max(1, min(self.max_chunk, min(j.steps - pool.status(jid)[1] for jid, j in active.items())))
+            if pending and step_s > 0.0:
                 gap = pending[0].arrival - (clock() - t0)
-                nstep = max(1, min(nstep, int(gap / self.step_s)))
+                nstep = max(1, min(nstep, int(gap / step_s)))
             ts = clock()
-            for k in busy:
-                self.pools[k].step(nstep)
-                launches += nstep
-            self.step_s = (clock() - ts) / nstep
+            pool.step(nstep)
+            launches += nstep
+            step_s = (clock() - ts) / nstep
             done_t = clock() - t0
-            for key in list(active):
-                k, jid = key
-                _, it, fin = self.pools[k].status(jid)
+            for jid in list(active):
+                _, it, fin = pool.status(jid)
                 if fin:
-                    j = active.pop(key)
+                    j = active.pop(jid)
                     if keep_results:
-                        j.result = self.pools[k].download(jid)
-                    self.pools[k].retire(jid)
+                        j.result = pool.download(jid)
+                    pool.retire(jid)
                     j.done_t = done_t
-        return summarize(jobs, launches)
+        return launches
 
 
 def summarize(jobs, launches=0):

@@ -80,7 +80,8 @@ def test_mhd3d_fp64_parity(oracle, rs, st, bc):
     _, R0 = smooth_state(oracle, ekp)
     g, o = run_pair(oracle, ekp, R0, 6)
     print(f"case=mhd3d 18x14x22 rs={rs} st={st} bc={bc}: max|diff|={np.abs(g - o).max():.3e}", flush=True)
-    assert rel(g, o) <= 1e-12
+    # FP64 parity is bitwise: same IEEE operations in the same order as the oracle
+    assert np.array_equal(g, o), rel(g, o)
 
 
 def test_mhd3d_blast_host_driven(oracle):
@@ -93,7 +94,7 @@ def test_mhd3d_blast_host_driven(oracle):
     finally:
         ctx.close()
     g, o = run_pair(oracle, ekp, R0, 5, use_step=True)
-    assert rel(g, o) <= 1e-12
+    assert np.array_equal(g, o), rel(g, o)
 
 
 def test_mhd3d_divb_conservation_64():
