@@ -53,7 +53,23 @@ UNITS = [
     ("ek_mhd3d_fp32.hip", FP32_FLAGS),
 ]
 HOST_UNITS = ["ConfigMap.cpp", "HydroParams.cpp", "SolverBase.cpp", "SolverHydroMuscl.cpp", "SolverMHDMuscl.cpp",
-              "SolverFactory.cpp", "Problems.cpp", "IO_VTK.cpp", "ek_host_capi.cpp"]
+              "SolverFactory.cpp", "Problems.cpp", "IO_VTK.cpp", "IO_HDF5.cpp", "ek_host_capi.cpp"]
+
+# Optional HDF5 output / restart: libek_h5.so (host/ek_h5.cpp) is built against the first
+# HDF5 installation found and opened by the host layer on first use.
+H5_LIB = os.path.join(PKG, "libek_h5.so")
+HDF5_DIRS = [d for d in (os.environ.get("EK_HDF5_DIR"), "/opt/conda", "/usr") if d]
+
+
+def find_hdf5():
+    for d in HDF5_DIRS:
+        for inc in (os.path.join(d, "include"), os.path.join(d, "include", "hdf5", "serial")):
+            if os.path.exists(os.path.join(inc, "hdf5.h")):
+                for lib in (os.path.join(d, "lib"), os.path.join(d, "lib", "x86_64-linux-gnu", "hdf5", "serial"),
+                            os.path.join(d, "lib", "x86_64-linux-gnu")):
+                    if any(os.path.exists(os.path.join(lib, n)) for n in ("libhdf5.so", "libhdf5.so.103")):
+                        return inc, lib
+    return None
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
               "-I", os.path.join(ROOT, "include"), "-I", HOST]
 
@@ -81,7 +97,14 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
         objs.append(o)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         list(ex.map(_run, cmds))
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB, *objs, "-ldl"])
+    h5 = find_hdf5()
+    if h5:
+        inc, lib = h5
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared", "-I", inc, "-o", H5_LIB,
+              os.path.join(HOST, "ek_h5.cpp"), f"-L{lib}", "-lhdf5", f"-Wl,-rpath,{lib}"])
+    elif os.path.exists(H5_LIB):
+        os.remove(H5_LIB)
     main_cpp = os.path.join(HOST, "main.cpp")
     if os.path.exists(main_cpp):
         _run(["g++", *HOST_FLAGS, "-o", EXE, main_cpp, LIB,

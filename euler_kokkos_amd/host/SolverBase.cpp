@@ -3,6 +3,11 @@
 
 #include <cmath>
 #include <cstdio>
+#include <iostream>
+#include <stdexcept>
+
+#include "IO_HDF5.h"
+#include "IO_VTK.h"
 
 namespace euler_kokkos {
 
@@ -93,6 +98,39 @@ void SolverBase::run()
 {
   while (!finished())
     next_iteration();
+}
+
+void SolverBase::save_data(const std::vector<double>& Usoa, const std::vector<double>& Uref)
+{
+  if (configMap.getBool("output", "vtk_enabled", true))
+    save_VTK(Usoa.data(), params, configMap, params.nbvar, m_variables_names, m_times_saved);
+  if (configMap.getBool("output", "hdf5_enabled", false))
+    save_HDF5(Usoa, Uref, params, configMap, m_variables_names, m_times_saved, m_t);
+}
+
+bool SolverBase::hdf5_needs_ghosts() const
+{
+  return configMap.getBool("output", "hdf5_enabled", false) && configMap.getBool("output", "ghostIncluded", false);
+}
+
+bool SolverBase::init_restart(std::vector<double>& Usoa)
+{
+  if (!configMap.getBool("run", "restart_enabled", false))
+    return false;
+  const std::string f = configMap.getString("run", "restart_filename", "");
+  const bool h5 = f.size() >= 3 && f.compare(f.size() - 3, 3, ".h5") == 0;
+  if (!configMap.getBool("output", "hdf5_enabled", false) || !h5)
+    throw std::runtime_error("restart needs [output] hdf5_enabled=true and an .h5 [run] restart_filename (got '" +
+                             f + "')");
+  int iStep = 0;
+  double t = 0.0;
+  load_HDF5(f, params, configMap, m_variables_names, Usoa, iStep, t);
+  m_times_saved = iStep;
+  m_t = t;
+  if (configMap.getBool("run", "restart_reset_totaltime", false))
+    m_t = 0;
+  std::cout << "### This is a restarted run ! Current time is " << m_t << " ###\n";
+  return true;
 }
 
 // src/shared/solver_utils.h:14-55

@@ -11,6 +11,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "ConfigMap.h"
 #include "HydroParams.h"
@@ -79,6 +80,19 @@ public:
   // on the device override it to run many iterations per host round trip when no host-side
   // event (log line, output file) falls between them.
   virtual void run();
+
+  // IO_ReadWrite::save_data (src/utils/io/IO_ReadWrite.cpp:74-168): VTK when [output]
+  // vtk_enabled (default true), HDF5 when [output] hdf5_enabled (default false).
+  // Usoa: interior SoA state; Uref: reference layout with ghosts, needed only when
+  // hdf5_needs_ghosts().
+  void save_data(const std::vector<double>& Usoa, const std::vector<double>& Uref);
+  bool hdf5_needs_ghosts() const;
+
+  // init_restart (src/muscl/SolverHydroMuscl.h:510-580, IO_ReadWrite::load_data_impl): with
+  // [run] restart_enabled, the state, m_times_saved and m_t come from the HDF5 file
+  // [run] restart_filename (which needs [output] hdf5_enabled and the .h5 suffix, as there);
+  // [run] restart_reset_totaltime sets m_t = 0. Returns false when no restart is asked for.
+  bool init_restart(std::vector<double>& Usoa);
 
   int m_times_saved;
   std::map<int, std::string> m_variables_names;

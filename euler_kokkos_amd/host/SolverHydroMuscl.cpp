@@ -5,7 +5,6 @@
 #include <iostream>
 #include <stdexcept>
 
-#include "IO_VTK.h"
 #include "Problems.h"
 
 namespace euler_kokkos {
@@ -32,7 +31,9 @@ SolverHydroMuscl<dim>::SolverHydroMuscl(HydroParams& params, ConfigMap& configMa
 
   // initial condition (SolverHydroMuscl.cpp:78-207); unknown problems fall back to implode
   std::vector<double> U;
-  if (!init_problem(m_problem_name, params, configMap, U)) {
+  if (init_restart(U)) {
+    // state, output counter and time from the restart file
+  } else if (!init_problem(m_problem_name, params, configMap, U)) {
     std::cout << "Problem : " << m_problem_name << " is not recognized / implemented." << std::endl;
     std::cout << "Use default - implode" << std::endl;
     init_implode(params, configMap, U);
@@ -119,6 +120,15 @@ void SolverHydroMuscl<dim>::run()
 }
 
 template <int dim>
+std::vector<double> SolverHydroMuscl<dim>::state_reference() const
+{
+  const long long n = (long long)params.isize * params.jsize * (dim == 3 ? params.ksize : 1) * params.nbvar;
+  std::vector<double> U((size_t)n);
+  check(ek_download(m_ctx, U.data(), EK_LAYOUT_REFERENCE), "ek_download");
+  return U;
+}
+
+template <int dim>
 std::vector<double> SolverHydroMuscl<dim>::state() const
 {
   const long long n = (long long)params.nx * params.ny * (dim == 3 ? params.nz : 1) * params.nbvar;
@@ -133,7 +143,7 @@ void SolverHydroMuscl<dim>::save_solution_impl()
 {
   timers[TIMER_IO]->start();
   const std::vector<double> U = state();
-  save_VTK(U.data(), params, configMap, params.nbvar, m_variables_names, m_times_saved);
+  save_data(U, hdf5_needs_ghosts() ? state_reference() : std::vector<double>());
   timers[TIMER_IO]->stop();
 }
 

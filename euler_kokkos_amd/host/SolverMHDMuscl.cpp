@@ -5,7 +5,6 @@
 #include <iostream>
 #include <stdexcept>
 
-#include "IO_VTK.h"
 #include "Problems.h"
 
 namespace euler_kokkos {
@@ -30,7 +29,9 @@ SolverMHDMuscl<dim>::SolverMHDMuscl(HydroParams& params, ConfigMap& configMap)
   if (ek_create(&ekp, &m_ctx) != EK_OK)
     throw std::runtime_error(std::string("ek_create: ") + ek_last_error(nullptr));
   std::vector<double> U;
-  if (!init_problem_mhd(m_problem_name, params, configMap, U)) {
+  if (init_restart(U)) {
+    // state, output counter and time from the restart file
+  } else if (!init_problem_mhd(m_problem_name, params, configMap, U)) {
     std::cout << "Problem : " << m_problem_name << " is not recognized / implemented." << std::endl;
     if (dim == 2) {
       std::cout << "Use default - Orszag-Tang vortex" << std::endl;
@@ -124,6 +125,15 @@ void SolverMHDMuscl<dim>::run()
 }
 
 template <int dim>
+std::vector<double> SolverMHDMuscl<dim>::state_reference() const
+{
+  const long long n = (long long)params.isize * params.jsize * (dim == 3 ? params.ksize : 1) * params.nbvar;
+  std::vector<double> U((size_t)n);
+  check(ek_download(m_ctx, U.data(), EK_LAYOUT_REFERENCE), "ek_download");
+  return U;
+}
+
+template <int dim>
 std::vector<double> SolverMHDMuscl<dim>::state() const
 {
   const long long n = (long long)params.nx * params.ny * (dim == 3 ? params.nz : 1) * params.nbvar;
@@ -132,13 +142,13 @@ std::vector<double> SolverMHDMuscl<dim>::state() const
   return U;
 }
 
-// save_solution_impl (src/muscl/SolverMHDMuscl.h:892-903): VTK output of the 8 variables
+// save_solution_impl (src/muscl/SolverMHDMuscl.h:892-903): VTK / HDF5 output of the 8 variables
 template <int dim>
 void SolverMHDMuscl<dim>::save_solution_impl()
 {
   timers[TIMER_IO]->start();
   const std::vector<double> U = state();
-  save_VTK(U.data(), params, configMap, params.nbvar, m_variables_names, m_times_saved);
+  save_data(U, hdf5_needs_ghosts() ? state_reference() : std::vector<double>());
   timers[TIMER_IO]->stop();
 }
 

@@ -35,6 +35,8 @@ public:
 
   // current state, SoA interior [var][k][j][i] (FP64)
   std::vector<double> state() const;
+  // current state in the reference layout U(i,j[,k],var) with the ghosts filled
+  std::vector<double> state_reference() const;
   ek_ctx* context() { return m_ctx; }
 
   int isize, jsize, ksize;

@@ -3,8 +3,8 @@
 //   euler_kokkos_amd settings.ini
 //
 // Reads the parameter file, builds the solver named by [run] solver_name through the
-// SolverFactory, writes the initial and final outputs when nOutput != 0, runs the time
-// loop and prints the timing summary (including the reference's Mcell-updates/s figure,
+// SolverFactory, writes the initial and final outputs when nOutput != 0 (VTK and/or HDF5,
+// plus the Xdmf wrapper of the HDF5 files), runs the time loop and prints the timing summary (including the reference's Mcell-updates/s figure,
 // which counts ghost cells, and the interior-cell rate).
 #include <cstdio>
 #include <iostream>
@@ -12,6 +12,7 @@
 
 #include "ConfigMap.h"
 #include "HydroParams.h"
+#include "IO_HDF5.h"
 #include "SolverFactory.h"
 
 int main(int argc, char* argv[])
@@ -37,6 +38,9 @@ int main(int argc, char* argv[])
     solver->timers[TIMER_TOTAL]->stop();
     if (params.nOutput != 0)
       solver->save_solution();
+    // Xdmf wrapper of the HDF5 outputs (main.cpp:140-147)
+    if (configMap.getBool("output", "hdf5_enabled", false))
+      ek::writeXdmfForHdf5Wrapper(params, configMap, solver->m_variables_names, solver->m_times_saved - 1, false);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     delete solver;

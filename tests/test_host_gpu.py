@@ -189,3 +189,125 @@ def test_ini_mhd3d_blast_matches_oracle(tmp_path, oracle):
     assert it == it_o == 6 and t == t_o
     want = U[3:-3, 3:-3, 3:-3, :].transpose(3, 2, 1, 0).reshape(-1)
     np.testing.assert_array_equal(G, want)
+
+
+# ---------------------------------------------------------------- HDF5 output and restart
+EXE = os.path.join(ROOT, "euler_kokkos_amd", "bin", "euler_kokkos_amd")
+H5LIB = os.path.join(ROOT, "euler_kokkos_amd", "libek_h5.so")
+needs_h5 = pytest.mark.skipif(not os.path.exists(H5LIB), reason="libek_h5.so not built (no HDF5 here)")
+
+
+def h5_read(path, shape, names):
+    L = C.CDLL(H5LIB)
+    rank = len(shape)
+    dims = (C.c_ulonglong * rank)(*shape)
+    nm = (C.c_char_p * len(names))(*[n.encode() for n in names])
+    outs = [np.zeros(shape) for _ in names]
+    ptrs = (C.POINTER(C.c_double) * len(outs))(*[o.ctypes.data_as(C.POINTER(C.c_double)) for o in outs])
+    step, t = C.c_int(), C.c_double()
+    err = C.create_string_buffer(512)
+    rc = L.ek_h5_read(str(path).encode(), rank, dims, len(names), nm, ptrs, C.byref(step), C.byref(t), err, 512)
+    assert rc == 0, err.value
+    return outs, step.value, t.value
+
+
+def run_exe(ini, cwd):
+    print(f"case=exe {ini.name}", flush=True)
+    r = subprocess.run([EXE, str(ini)], capture_output=True, text=True, timeout=300, cwd=str(cwd))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+def h5_files(d):
+    return sorted(p for p in os.listdir(d) if p.endswith(".h5"))
+
+
+HYDRO_NAMES = ["rho", "energy", "rho_vx", "rho_vy"]
+
+
+@needs_h5
+def test_hdf5_output_ghosts_and_restart_continue_bitwise(tmp_path):
+    """Save_HDF5 / Load_HDF5 through the executable: interior files, ghost-included files
+    whose interior is the same and whose ghosts follow the boundary rules, and a restart from
+    an intermediate file (with or without ghosts) that ends bitwise on the uninterrupted run."""
+    base = INI_2D.replace("nOutput=0", "nOutput=4")
+    out = {}
+    for name, extra in (("a", ""), ("g", "ghostIncluded=true\n")):
+        d = tmp_path / name
+        d.mkdir()
+        ini = tmp_path / f"{name}.ini"
+        ini.write_text(base + f"[output]\noutputDir={d}\noutputPrefix=blast\nvtk_enabled=false\nhdf5_enabled=true\n"
+                       + extra)
+        run_exe(ini, tmp_path)
+        out[name] = h5_files(d)
+    assert out["a"] == out["g"] and len(out["a"]) >= 5
+    xmf = (tmp_path / "blast.xmf").read_text()
+    assert xmf.count('GridType="Uniform"') == len(out["a"]) and "blast_0000002.h5:/rho_vx" in xmf
+    for f in out["a"]:
+        A, sa, ta = h5_read(tmp_path / "a" / f, (64, 96), HYDRO_NAMES)
+        G, sg, tg = h5_read(tmp_path / "g" / f, (68, 100), HYDRO_NAMES)
+        assert sa == sg and ta == tg
+        for a, g in zip(A, G):
+            assert np.array_equal(g[2:-2, 2:-2], a)
+        # x min is reflecting: ghost column i = 1 mirrors i = 2, normal momentum negated
+        assert np.array_equal(G[0][:, 1], G[0][:, 2]) and np.array_equal(G[2][:, 1], -G[2][:, 2])
+        # y min/max periodic
+        assert np.array_equal(G[0][0:2, 2:-2], G[0][64:66, 2:-2])
+    last = out["a"][-1]
+    A_end, s_end, t_end = h5_read(tmp_path / "a" / last, (64, 96), HYDRO_NAMES)
+    for src in ("a", "g"):
+        d = tmp_path / f"r{src}"
+        d.mkdir()
+        ini = tmp_path / f"r{src}.ini"
+        ghost = "ghostIncluded=true\n" if src == "g" else ""
+        ini.write_text(base.replace("[run]\n", f"[run]\nrestart_enabled=1\nrestart_filename={tmp_path / src / out[src][2]}\n")
+                       + f"[output]\noutputDir={d}\noutputPrefix=blast\nvtk_enabled=false\nhdf5_enabled=true\n" + ghost)
+        log = run_exe(ini, tmp_path)
+        assert "This is a restarted run" in log
+        files = h5_files(d)
+        assert files[0] == out[src][2] and files[-1] == last
+        shape = (68, 100) if src == "g" else (64, 96)
+        R, s, t = h5_read(d / last, shape, HYDRO_NAMES)
+        assert s == s_end and t == t_end
+        for r, a in zip(R, A_end):
+            assert np.array_equal(r[2:-2, 2:-2] if src == "g" else r, a)
+
+
+@needs_h5
+def test_hdf5_restart_upscale_mhd(tmp_path):
+    """[run] restart_upscale: a 30x22 Orszag-Tang file (ghosts included) restarts a 60x44 run;
+    every fine cell takes its coarse cell, and a fine face between two coarse faces takes
+    their mean (Load_HDF5::copy_buffer, IO_HDF5.h:1593-1650)."""
+    names = ["rho", "energy", "rho_vx", "rho_vy", "rho_vz", "bx", "by", "bz"]
+    coarse = INI_MHD.replace("nx=60", "nx=30").replace("ny=44", "ny=22").replace("nStepmax=25", "nStepmax=3") \
+                    .replace("nOutput=0", "nOutput=1")
+    dc = tmp_path / "c"
+    dc.mkdir()
+    ini = tmp_path / "c.ini"
+    ini.write_text(coarse + f"[output]\noutputDir={dc}\noutputPrefix=ot\nvtk_enabled=false\nhdf5_enabled=true\n"
+                   "ghostIncluded=true\n")
+    run_exe(ini, tmp_path)
+    cf = h5_files(dc)[-1]
+    Cv, cstep, ct = h5_read(dc / cf, (28, 36), names)
+    df = tmp_path / "f"
+    df.mkdir()
+    ini = tmp_path / "f.ini"
+    ini.write_text(INI_MHD.replace("nStepmax=25", "nStepmax=0").replace("nOutput=0", "nOutput=1")
+                   .replace("[run]\n", f"[run]\nrestart_enabled=1\nrestart_filename={dc / cf}\nrestart_upscale=true\n")
+                   + f"[output]\noutputDir={df}\noutputPrefix=ot\nvtk_enabled=false\nhdf5_enabled=true\n"
+                   "ghostIncluded=true\n")
+    run_exe(ini, tmp_path)
+    F, fstep, ft = h5_read(df / h5_files(df)[0], (50, 66), names)
+    assert fstep == cstep and ft == ct
+    gw = 3
+    J, I = np.meshgrid(np.arange(gw, 44 + gw), np.arange(gw, 60 + gw), indexing="ij")
+    jl, il = (J + gw) // 2, (I + gw) // 2
+    for v, (c, f) in enumerate(zip(Cv, F)):
+        want = c[jl, il]
+        if names[v] == "bx":
+            odd = (I + gw - 2 * il) != 0
+            want = np.where(odd, (c[jl, il] + c[jl, np.minimum(il + 1, 35)]) / 2, want)
+        if names[v] == "by":
+            odd = (J + gw - 2 * jl) != 0
+            want = np.where(odd, (c[jl, il] + c[np.minimum(jl + 1, 27), il]) / 2, want)
+        np.testing.assert_array_equal(f[gw:-gw, gw:-gw], want, err_msg=names[v])
