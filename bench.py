@@ -95,6 +95,68 @@ def cpu_baseline(n, seconds):
             "ref_style_mcell_per_s": steps * (n + 4) ** 3 / el * 1e-6}
 
 
+def mhd3d_c5(n, steps, with_cpu, cpu_seconds):
+    """3D ideal-MHD blast (the serving stream's MHD job kind) on n^3, periodic, HLLD + CT,
+    gamma 1.6666667, CFL 0.8, MC slopes, field B0 = 0.5 along (1,1,1): the six-kernel step
+    pipeline in FP64 and FP32, the discrete max |div B| of the FP64 result, and the oracle
+    on the host cores."""
+    from euler_kokkos_amd import native, problems
+    from euler_kokkos_amd.serving import job_params
+    res = {"workload": f"3D MHD blast {n}^3 periodic, HLLD + CT, MC slopes, gamma 5/3, CFL 0.8, "
+                       f"B0 0.5 along (1,1,1), {steps} steps", "grid": [n, n, n], "solver": "MHD_Muscl_3D"}
+    out = {}
+    U0 = None
+    for name, prec in (("fp64", native.EK_FP64), ("fp32", native.EK_FP32)):
+        ekp = job_params("mhd_blast", n, prec)
+        if U0 is None:
+            U0 = problems.mhd_blast3d(ekp, B0=0.5)
+        ctx = native.Context(ekp)
+        try:
+            ctx.upload(U0, native.EK_LAYOUT_SOA)
+            b0 = ctx.budgets()
+            ctx.run(2, 2, 1e30)
+            device_sync()
+            t0 = time.perf_counter()
+            ctx.run(steps, steps + 2, 1e30)
+            device_sync()
+            wall = time.perf_counter() - t0
+            ev_ms, launches = ctx.last_run_time()
+            b1 = ctx.budgets()
+            U = ctx.download(native.EK_LAYOUT_SOA)
+        finally:
+            ctx.close()
+        A, B, C = U[5], U[6], U[7]
+        divb = np.abs((np.roll(A, -1, axis=2) - A) + (np.roll(B, -1, axis=1) - B) + (np.roll(C, -1, axis=0) - C)).max()
+        out[name] = U
+        res[name] = {"value": steps * n ** 3 / wall, "unit": "cell-updates/s", "ms_per_step": wall / steps * 1e3,
+                     "ms_per_step_events": ev_ms / max(launches, 1),
+                     "achieved_GBps": n ** 3 * 2 * 8 * (8 if prec == native.EK_FP64 else 4)
+                     / (ev_ms / max(launches, 1) * 1e-3) / 1e9,
+                     "max_abs_divB_times_dx": float(divb),
+                     "mass_drift_rel": float(abs(b1[0] - b0[0]) / abs(b0[0])),
+                     "energy_drift_rel": float(abs(b1[1] - b0[1]) / abs(b0[1]))}
+    res["fp32"]["l1_rho_rel"] = float(np.abs(out["fp32"][0] - out["fp64"][0]).mean() / np.abs(out["fp64"][0]).mean())
+    if with_cpu:
+        from oracle import pyoracle
+        ekp = job_params("mhd_blast", n, native.EK_FP64)
+        op = pyoracle.from_ek(ekp)
+        R = np.zeros(pyoracle.ref_shape(op))
+        g = op.gw
+        R[g:-g, g:-g, g:-g, :] = U0.transpose(3, 2, 1, 0)
+        t0 = time.perf_counter()
+        pyoracle.mhd3d_run(op, R, 1, 1e30)
+        one = time.perf_counter() - t0
+        k = int(max(1, min(20, cpu_seconds // max(one, 1e-3))))
+        t0 = time.perf_counter()
+        pyoracle.mhd3d_run(op, R, k + 1, 1e30, 0.0, 1)
+        el = time.perf_counter() - t0
+        cores = pyoracle.lib().eko_num_threads()
+        res["cpu_baseline"] = {"value": k * n ** 3 / el, "unit": "cell-updates/s", "cores": int(cores),
+                               "kind": "port", "sample": f"oracle 3D MHD step on the same {n}^3 blast, "
+                               f"{k} steps after 1 untimed step, {cores} OpenMP threads"}
+    return res
+
+
 def mhd_c3(steps, with_cpu, cpu_seconds):
     """2D ideal-MHD Orszag-Tang vortex, 1024^2, periodic, HLLD + constrained transport,
     gamma 1.6666667 (float-parsed), CFL 0.8, MC slopes: FP64 and FP32 bulk on the GPU, the
@@ -285,6 +347,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-mhd", action="store_true", help="skip the 2D MHD (Orszag-Tang) line")
     ap.add_argument("--mhd-steps", type=int, default=500)
+    ap.add_argument("--no-mhd3d", action="store_true", help="skip the 3D MHD (blast) line")
+    ap.add_argument("--mhd3d-n", type=int, default=128)
+    ap.add_argument("--mhd3d-steps", type=int, default=30)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched 2D hydro line")
     ap.add_argument("--batch-runs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100)
@@ -355,6 +420,8 @@ def main() -> int:
         out["cpu_baseline"] = None
     if not a.no_mhd:
         out["mhd2d"] = mhd_c3(a.mhd_steps, not a.no_cpu, a.cpu_seconds / 2)
+    if not a.no_mhd3d:
+        out["mhd3d"] = mhd3d_c5(a.mhd3d_n, a.mhd3d_steps, not a.no_cpu, a.cpu_seconds / 2)
     if not a.no_batch:
         out["batch2d"] = batch_c4(a.batch_runs, a.batch_steps, a.batch_seed)
     if not a.no_serve:
