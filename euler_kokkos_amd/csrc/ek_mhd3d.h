@@ -50,20 +50,62 @@ enum {
 // (x: IB IC; y: IA IC; z: IA IB).
 enum { K_C = 0, K_AL = 8, K_AR, K_BL, K_BR, K_CL, K_CR, K_X = 14, K_Y = 21, K_Z = 28, K_N = 35 };
 
-__device__ __forceinline__ long long pidx(const Work3& w, int i, int j, int k)
+// padded linear index (below 2^30: the host keeps every array below 4 GiB)
+__device__ __forceinline__ unsigned pidx(const Work3& w, int i, int j, int k)
 {
-  return ((long long)k * w.PY + j) * w.PX + i;
+  return ((unsigned)k * (unsigned)w.PY + (unsigned)j) * (unsigned)w.PX + (unsigned)i;
+}
+
+// Element at a 32-bit byte offset from a wave-uniform array base: the access addresses as a
+// scalar base + one 32-bit lane offset per cell, instead of a 64-bit address per (array,
+// cell), which takes two registers and 64-bit adds each. Each array of one grid is below
+// 4 GiB (checked by the host). The base is the same in every lane; passing it through
+// readfirstlane keeps the compiler from folding the array offset into a 64-bit per-lane
+// address. Used where it measured faster: the record reads of the flux/EMF kernel (fewer
+// instructions and registers) and the prims and update kernels; the pad, E-field and trace
+// kernels keep plain indexing, with which the compiler spaces their loads better.
+typedef char __attribute__((address_space(1))) * gbytes_t;
+
+template <class T>
+__device__ __forceinline__ gbytes_t sbase(const T* base)
+{
+  const unsigned long long a = reinterpret_cast<unsigned long long>(base);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  return (gbytes_t)(((unsigned long long)hi << 32) | lo);
+}
+
+template <class T>
+__device__ __forceinline__ T ldb(const T* base, unsigned boff)
+{
+  return *(const T __attribute__((address_space(1)))*)(sbase(base) + boff);
+}
+
+template <class T>
+__device__ __forceinline__ void stb(T* base, unsigned boff, T x)
+{
+  *(T __attribute__((address_space(1)))*)(sbase(base) + boff) = x;
+}
+
+// (i, j, k) of linear index c in a box of X x Y x (any) cells. 32-bit division: the host
+// keeps every work-area array below 4 GiB, so every index here is below 2^30 (a 64-bit
+// division would cost each thread a long software sequence).
+__device__ __forceinline__ void split3(unsigned c, unsigned X, unsigned Y, int& i, int& j, int& k)
+{
+  const unsigned r = c / X;
+  const unsigned z = r / Y;
+  i = (int)(c - r * X);
+  j = (int)(r - z * Y);
+  k = (int)z;
 }
 
 template <class T>
 __device__ __forceinline__ bool cell_of(const Work3& w, int& i, int& j, int& k)
 {
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= w.PN)
+  const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= (unsigned long long)w.PN)
     return false;
-  i = (int)(c % w.PX);
-  j = (int)((c / w.PX) % w.PY);
-  k = (int)(c / ((long long)w.PX * w.PY));
+  split3(c, (unsigned)w.PX, (unsigned)w.PY, i, j, k);
   return true;
 }
 
@@ -120,16 +162,18 @@ __global__ __launch_bounds__(256) void prims_kernel(T* __restrict__ S, Work3 w, 
   int i, j, k;
   if (!cell_of<T>(w, i, j, k) || i >= w.PX - 1 || j >= w.PY - 1 || k >= w.PZ - 1)
     return;
-  const long long po = pidx(w, i, j, k), PN = w.PN;
+  const long long PN = w.PN;
+  const unsigned pb = pidx(w, i, j, k) * (unsigned)sizeof(T);
   T u[NV], q[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++)
-    u[v] = S[(A_UP + v) * PN + po];
-  constoprim<T>(gas, u, S[(A_UP + IA) * PN + po + 1], S[(A_UP + IB) * PN + po + w.PX],
-                S[(A_UP + IC) * PN + po + (long long)w.PX * w.PY], q);
+    u[v] = ldb(S + (A_UP + v) * PN, pb);
+  constoprim<T>(gas, u, ldb(S + (A_UP + IA) * PN, pb + (unsigned)sizeof(T)),
+                ldb(S + (A_UP + IB) * PN, pb + (unsigned)w.PX * (unsigned)sizeof(T)),
+                ldb(S + (A_UP + IC) * PN, pb + (unsigned)(w.PX * w.PY) * (unsigned)sizeof(T)), q);
 #pragma unroll
   for (int v = 0; v < NV; v++)
-    S[(A_Q + v) * PN + po] = q[v];
+    stb(S + (A_Q + v) * PN, pb, q[v]);
 }
 
 // edge electric fields and transverse face-field slopes on 1 <= i,j,k < size-1
@@ -288,16 +332,6 @@ __global__ __launch_bounds__(256) void trace_kernel(T* __restrict__ S, Work3 w, 
   put(K_Z + 6, dBz);
 }
 
-// Element at a 32-bit byte offset from a wave-uniform array base: the record and slope
-// reads below then address as scalar base + one 32-bit lane offset per cell, instead of a
-// 64-bit address per (array, cell), which would need two registers each. The work area
-// of one grid is below 4 GiB per array (checked by the host).
-template <class T>
-__device__ __forceinline__ T ldb(const T* base, unsigned boff)
-{
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
-}
-
 template <class T>
 struct RecView {
   const T* S;
@@ -398,27 +432,28 @@ __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w
     return;
   const unsigned b = blockIdx.x;
   const int task = (b >> 3) & 3;
-  const long long c = ((long long)(((b >> 5) << 3) | (b & 7))) * 256 + threadIdx.x;
-  if (c >= w.PN)
+  const unsigned c = (((b >> 5) << 3) | (b & 7)) * 256u + threadIdx.x;
+  if (c >= (unsigned long long)w.PN)
     return;
-  const int i = (int)(c % w.PX), j = (int)((c / w.PX) % w.PY), k = (int)(c / ((long long)w.PX * w.PY));
+  int i, j, k;
+  split3(c, (unsigned)w.PX, (unsigned)w.PY, i, j, k);
   if (i < 3 || j < 3 || k < 3 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
     return;
-  const long long PN = w.PN, o = c;
-  const unsigned ob = (unsigned)o * (unsigned)sizeof(T), bx = sizeof(T), by = (unsigned)w.PX * (unsigned)sizeof(T);
+  const long long PN = w.PN;
+  const unsigned ob = c * (unsigned)sizeof(T), bx = sizeof(T), by = (unsigned)w.PX * (unsigned)sizeof(T);
   const unsigned bz = (unsigned)(w.PX * w.PY) * (unsigned)sizeof(T);
   const RecView<T> R{ S, PN };
   if (task != 0) {
     auto E = [&](unsigned cb, int e, int d) { return edge_state<T>(R, cb, e, d, bx, by, bz, gas); };
     if (task == 1) {
       // EMF z: RT(i-1,j-1,k), RB(i-1,j,k), LT(i,j-1,k), LB(i,j,k) of the Z edges
-      S[(A_EMF + 2) * PN + o] = emf_z<T>(gas, E(ob - bx - by, 0, 2), E(ob - bx, 1, 2), E(ob - by, 2, 2), E(ob, 3, 2));
+      S[(A_EMF + 2) * PN + c] = emf_z<T>(gas, E(ob - bx - by, 0, 2), E(ob - bx, 1, 2), E(ob - by, 2, 2), E(ob, 3, 2));
     } else if (task == 2) {
       // EMF y: RT(i-1,j,k-1), LT(i,j,k-1) in the RB place, RB(i-1,j,k) in the LT place, LB(i,j,k)
-      S[(A_EMF + 1) * PN + o] = emf_z<T>(gas, E(ob - bx - bz, 0, 1), E(ob - bz, 2, 1), E(ob - bx, 1, 1), E(ob, 3, 1));
+      S[(A_EMF + 1) * PN + c] = emf_z<T>(gas, E(ob - bx - bz, 0, 1), E(ob - bz, 2, 1), E(ob - bx, 1, 1), E(ob, 3, 1));
     } else {
       // EMF x: RT(i,j-1,k-1), RB(i,j-1,k), LT(i,j,k-1), LB(i,j,k)
-      S[(A_EMF + 0) * PN + o] = emf_z<T>(gas, E(ob - by - bz, 0, 0), E(ob - by, 1, 0), E(ob - bz, 2, 0), E(ob, 3, 0));
+      S[(A_EMF + 0) * PN + c] = emf_z<T>(gas, E(ob - by - bz, 0, 0), E(ob - by, 1, 0), E(ob - bz, 2, 0), E(ob, 3, 0));
     }
     return;
   }
@@ -428,7 +463,7 @@ __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w
   face_state<T>(R, ob, 0, false, gas, qr);
   riemann<T, RS>(gas, ql, qr, f);
   {
-    T* F = S + (A_FLUX + 0) * PN + o;
+    T* F = S + (A_FLUX + 0) * PN + c;
     F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IU]; F[3 * PN] = f[IV]; F[4 * PN] = f[IW];
   }
   // y: qm_y(j-1) | qp_y(j), IU<->IV and IA<->IB around the solve
@@ -443,7 +478,7 @@ __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w
   }
   riemann<T, RS>(gas, ql, qr, f);
   {
-    T* F = S + (A_FLUX + 5) * PN + o;
+    T* F = S + (A_FLUX + 5) * PN + c;
     F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IV]; F[3 * PN] = f[IU]; F[4 * PN] = f[IW];
   }
   // z: qm_z(k-1) | qp_z(k), IU<->IW and IA<->IC around the solve
@@ -458,7 +493,7 @@ __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w
   }
   riemann<T, RS>(gas, ql, qr, f);
   {
-    T* F = S + (A_FLUX + 10) * PN + o;
+    T* F = S + (A_FLUX + 10) * PN + c;
     F[0] = f[ID]; F[PN] = f[IP]; F[2 * PN] = f[IW]; F[3 * PN] = f[IV]; F[4 * PN] = f[IU];
   }
 }
@@ -523,60 +558,66 @@ __global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T*
   double local = 0.0;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
        c += (long long)gridDim.x * blockDim.x) {
-    const int ii = (int)(c % w.nx), jj = (int)((c / w.nx) % w.ny), kk = (int)(c / ((long long)w.nx * w.ny));
-    const long long PN = w.PN, o = pidx(w, ii + 3, jj + 3, kk + 3);
-    const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
+    int ii, jj, kk;
+    split3((unsigned)c, (unsigned)w.nx, (unsigned)w.ny, ii, jj, kk);
+    const long long PN = w.PN;
+    // byte offsets: this cell of the padded grid, its strides, this cell of the output
+    const unsigned o = pidx(w, ii + 3, jj + 3, kk + 3) * (unsigned)sizeof(T);
+    const unsigned sx = sizeof(T), sy = (unsigned)w.PX * (unsigned)sizeof(T);
+    const unsigned sz = (unsigned)(w.PX * w.PY) * (unsigned)sizeof(T);
+    const unsigned ob = (unsigned)c * (unsigned)sizeof(T);
+    auto at = [&](int a, unsigned off) -> T { return ldb(S + a * PN, off); };
     T u[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++)
-      u[v] = S[(A_UP + v) * PN + o];
+      u[v] = at(A_UP + v, o);
     constexpr int hv[5] = { ID, IP, IU, IV, IW };
 #pragma unroll
     for (int m = 0; m < 5; m++) {
       T a = u[hv[m]];
-      a += S[(A_FLUX + m) * PN + o] * dtdx;
-      a -= S[(A_FLUX + m) * PN + o + sx] * dtdx;
-      a += S[(A_FLUX + 5 + m) * PN + o] * dtdy;
-      a -= S[(A_FLUX + 5 + m) * PN + o + sy] * dtdy;
-      a += S[(A_FLUX + 10 + m) * PN + o] * dtdz;
-      a -= S[(A_FLUX + 10 + m) * PN + o + sz] * dtdz;
+      a += at(A_FLUX + m, o) * dtdx;
+      a -= at(A_FLUX + m, o + sx) * dtdx;
+      a += at(A_FLUX + 5 + m, o) * dtdy;
+      a -= at(A_FLUX + 5 + m, o + sy) * dtdy;
+      a += at(A_FLUX + 10 + m, o) * dtdz;
+      a -= at(A_FLUX + 10 + m, o + sz) * dtdz;
       u[hv[m]] = a;
     }
-    const T* EX = S + (A_EMF + 0) * PN;
-    const T* EY = S + (A_EMF + 1) * PN;
-    const T* EZ = S + (A_EMF + 2) * PN;
-    u[IA] += (EZ[o + sy] - EZ[o]) * dtdy;
-    u[IB] -= (EZ[o + sx] - EZ[o]) * dtdx;
-    u[IA] -= (EY[o + sz] - EY[o]) * dtdz;
-    u[IB] += (EX[o + sz] - EX[o]) * dtdz;
-    u[IC] += (EY[o + sx] - EY[o]) * dtdx;
-    u[IC] -= (EX[o + sy] - EX[o]) * dtdy;
+    auto EX = [&](unsigned off) -> T { return at(A_EMF + 0, off); };
+    auto EY = [&](unsigned off) -> T { return at(A_EMF + 1, off); };
+    auto EZ = [&](unsigned off) -> T { return at(A_EMF + 2, off); };
+    u[IA] += (EZ(o + sy) - EZ(o)) * dtdy;
+    u[IB] -= (EZ(o + sx) - EZ(o)) * dtdx;
+    u[IA] -= (EY(o + sz) - EY(o)) * dtdz;
+    u[IB] += (EX(o + sz) - EX(o)) * dtdz;
+    u[IC] += (EY(o + sx) - EY(o)) * dtdx;
+    u[IC] -= (EX(o + sy) - EX(o)) * dtdy;
 #pragma unroll
     for (int v = 0; v < NV; v++)
-      Uout[v * n + c] = u[v];
+      stb(Uout + v * n, ob, u[v]);
     // new face fields of the right / top / front neighbours, as the next ghost fill sees them
     T ar, bt, cf;
     if (ii + 1 < w.nx || geo.bc[1] == BC_PERIODIC) {
-      const long long q = o + sx;
-      ar = S[(A_UP + IA) * PN + q];
-      ar += (EZ[q + sy] - EZ[q]) * dtdy;
-      ar -= (EY[q + sz] - EY[q]) * dtdz;
+      const unsigned q = o + sx;
+      ar = at(A_UP + IA, q);
+      ar += (EZ(q + sy) - EZ(q)) * dtdy;
+      ar -= (EY(q + sz) - EY(q)) * dtdz;
     } else {
       ar = geo.bc[1] == BC_DIRICHLET ? u[IA] * T(-1) : u[IA];
     }
     if (jj + 1 < w.ny || geo.bc[3] == BC_PERIODIC) {
-      const long long q = o + sy;
-      bt = S[(A_UP + IB) * PN + q];
-      bt -= (EZ[q + sx] - EZ[q]) * dtdx;
-      bt += (EX[q + sz] - EX[q]) * dtdz;
+      const unsigned q = o + sy;
+      bt = at(A_UP + IB, q);
+      bt -= (EZ(q + sx) - EZ(q)) * dtdx;
+      bt += (EX(q + sz) - EX(q)) * dtdz;
     } else {
       bt = geo.bc[3] == BC_DIRICHLET ? u[IB] * T(-1) : u[IB];
     }
     if (kk + 1 < w.nz || geo.bc[5] == BC_PERIODIC) {
-      const long long q = o + sz;
-      cf = S[(A_UP + IC) * PN + q];
-      cf += (EY[q + sx] - EY[q]) * dtdx;
-      cf -= (EX[q + sy] - EX[q]) * dtdy;
+      const unsigned q = o + sz;
+      cf = at(A_UP + IC, q);
+      cf += (EY(q + sx) - EY(q)) * dtdx;
+      cf -= (EX(q + sy) - EX(q)) * dtdy;
     } else {
       cf = geo.bc[5] == BC_DIRICHLET ? u[IC] * T(-1) : u[IC];
     }
@@ -599,7 +640,8 @@ __global__ __launch_bounds__(256) void inv_dt3_kernel(const T* __restrict__ U, G
   const int nx = geo.nx, ny = geo.ny, nz = geo.nz;
   double local = 0.0;
   for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
-    const int i = (int)(c % nx), j = (int)((c / nx) % ny), k = (int)(c / ((long long)nx * ny));
+    int i, j, k;
+    split3((unsigned)c, (unsigned)nx, (unsigned)ny, i, j, k);
     T u[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++)
