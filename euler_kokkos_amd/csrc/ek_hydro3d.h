@@ -167,8 +167,9 @@ __device__ __forceinline__ constexpr int rot_z(int v)
 
 // One TX x TY column tile over planes [k0, k0+CZ) of the 3D step with time step dt.
 // Returns this thread's contribution to the CFL maximum of the new state. Shared by the
-// single-grid kernel below and the batched (job pool) kernel.
-template <class T, int TX, int TY, int NT, int RS, int ST>
+// single-grid kernel below and the batched (job pool) kernel. EARLY: the owner reads its
+// U(p) before the Riemann phase instead of in the update (below).
+template <class T, int TX, int TY, int NT, int RS, int ST, bool EARLY = (sizeof(T) == 8)>
 __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* __restrict__ Uout,
                                                  const Geom<T>& geo, const Gas<T>& gas, double dt, int i0,
                                                  int j0, int k0, int CZ)
@@ -327,6 +328,18 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
       for (int v = 0; v < NV; v++)
         sQ[v][c] = qn[v];
     }
+    // EARLY (the FP64 single-grid step): the owner's U(p) is read here, so the load
+    // completes behind the Riemann phase; the 10 registers it holds through C stay under the
+    // 3-wave limit that the LDS footprint sets anyway (in FP32 they would cost a wave, and
+    // the pool kernel, with its job records, would spill)
+    if constexpr (EARLY) {
+      if (owner && !warm && !last && ovalid) {
+        const long long o = (long long)p * geo.plane + ocol;
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+          ucur[v] = Uin[v * geo.vstride + o];
+      }
+    }
     // C: Riemann problems of plane p's faces: own z face, then this wave's x/y range
     if (!warm) {
       if (owner) {
@@ -381,11 +394,13 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
       if (!last) {
         // the owner's U(p), read late to keep it out of the register peak of B and C (this
         // thread streamed the same column two planes ago, so the read hits in cache)
-        if (ovalid) {
-          const long long o = (long long)p * geo.plane + ocol;
+        if constexpr (!EARLY) {
+          if (ovalid) {
+            const long long o = (long long)p * geo.plane + ocol;
 #pragma unroll
-          for (int v = 0; v < NV; v++)
-            ucur[v] = Uin[v * geo.vstride + o];
+            for (int v = 0; v < NV; v++)
+              ucur[v] = Uin[v * geo.vstride + o];
+          }
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) {

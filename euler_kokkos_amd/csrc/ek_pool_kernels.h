@@ -104,7 +104,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3))) void hy
     if (!step_begin(ctl, first, dt, cur))
       continue;
     __syncthreads();
-    const double local = hydro3d_column<T, TX, TY, NT, RS, ST>(Ua + J.off, Ub + J.off, J.geo, J.gas, dt,
+    const double local = hydro3d_column<T, TX, TY, NT, RS, ST, false>(Ua + J.off, Ub + J.off, J.geo, J.gas, dt,
                                                                it.tx * TX, it.ty * TY, (it.tz_first >> 1) * J.cz,
                                                                J.cz);
     block_max_to<NT>(&ctl.ts[(slot + 1) % 3].invdt_bits, local, sred);
