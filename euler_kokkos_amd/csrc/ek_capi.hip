@@ -35,14 +35,18 @@ struct ek_ctx {
   int st = 2;
   int cz = 32;
   long long bytes = 0;
-  // captured 6-step graph (6 = lcm of the 2 state buffers and the 3 time slots)
-  hipGraphExec_t graph = nullptr;
+  // captured 6-step graphs (6 = lcm of the 2 state buffers and the 3 time slots), one per
+  // physical buffer the run starts from: a run of an odd number of steps leaves the state in
+  // the other buffer, and the next run then replays the other graph instead of capturing anew
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    int nStepmax = -1;
+    double tEnd = 0.0;
+    void* U0 = nullptr;
+  } graphs[2];
   hipEvent_t ev0 = nullptr, ev1 = nullptr; // bracket the step launches of the last ek_run
   double last_run_ms = 0.0;
   int last_run_launches = 0;
-  int graph_nStepmax = -1;
-  double graph_tEnd = 0.0;
-  void* graph_U0 = nullptr;
 };
 
 static std::string g_create_err;
@@ -182,8 +186,9 @@ extern "C" void ek_destroy(ek_ctx* c)
     return;
   if (c->stream)
     hipStreamSynchronize(c->stream);
-  if (c->graph)
-    hipGraphExecDestroy(c->graph);
+  for (auto& g : c->graphs)
+    if (g.exec)
+      hipGraphExecDestroy(g.exec);
   if (c->ev0)
     hipEventDestroy(c->ev0);
   if (c->ev1)
@@ -374,11 +379,14 @@ extern "C" int ek_step(ek_ctx* c, double dt)
   return EK_OK;
 }
 
-static int build_graph(ek_ctx* c, int nStepmax, double tEnd)
+// (Re)captures graphs[s] for a run that starts from the current U[0].
+static int build_graph(ek_ctx* c, int s, int nStepmax, double tEnd)
 {
-  if (c->graph) {
-    hipGraphExecDestroy(c->graph);
-    c->graph = nullptr;
+  ek_ctx::Graph& G = c->graphs[s];
+  if (G.exec) {
+    hipGraphExecDestroy(G.exec);
+    G.exec = nullptr;
+    G.U0 = nullptr;
   }
   hipGraph_t g = nullptr;
   EK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
@@ -400,13 +408,15 @@ static int build_graph(ek_ctx* c, int nStepmax, double tEnd)
     }
   }
   EK_HIP(c, hipStreamEndCapture(c->stream, &g));
-  hipError_t e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
   hipGraphDestroy(g);
-  if (e != hipSuccess)
+  if (e != hipSuccess) {
+    G.exec = nullptr;
     return hip_fail(c, e, "hipGraphInstantiate");
-  c->graph_nStepmax = nStepmax;
-  c->graph_tEnd = tEnd;
-  c->graph_U0 = c->U[0];
+  }
+  G.nStepmax = nStepmax;
+  G.tEnd = tEnd;
+  G.U0 = c->U[0];
   return EK_OK;
 }
 
@@ -422,6 +432,13 @@ extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* 
     std::swap(c->U[0], c->U[1]);
     c->cur = 0;
   }
+  // the graph of this start buffer: the one captured from it, else the slot not holding the
+  // other buffer's graph
+  const int gs = c->graphs[0].U0 == c->U[0] ? 0 : (c->graphs[1].U0 == c->U[0] ? 1 : (c->graphs[0].U0 ? 1 : 0));
+  const ek_ctx::Graph& G = c->graphs[gs];
+  const bool stale = !G.exec || G.nStepmax != nStepmax || G.tEnd != tEnd || G.U0 != c->U[0];
+  if (nsteps == 0) // warm-up call: capture only
+    return stale ? build_graph(c, gs, nStepmax, tEnd) : EK_OK;
   ek::TimeSlot init[3];
   std::memset(init, 0, sizeof(init));
   init[0].t = *t_io;
@@ -430,14 +447,13 @@ extern "C" int ek_run(ek_ctx* c, int nsteps, int nStepmax, double tEnd, double* 
   EK_HIP(c, hipMemsetAsync(c->spread, 0, ek::SPREAD_ELEMS * sizeof(unsigned long long), c->stream));
   EK_HIP(c, launch_inv_dt_into(c, &c->ts[0].invdt_bits, c->spread)); // slot 0's words
   const int nfull = nsteps / 6, rem = nsteps % 6;
-  if (nfull > 0 && (!c->graph || c->graph_nStepmax != nStepmax || c->graph_tEnd != tEnd ||
-                    c->graph_U0 != c->U[0]))
-    if (int r = build_graph(c, nStepmax, tEnd))
+  if (nfull > 0 && stale)
+    if (int r = build_graph(c, gs, nStepmax, tEnd))
       return r;
   EK_HIP(c, hipEventRecord(c->ev0, c->stream));
   if (nfull > 0) {
     for (int g = 0; g < nfull; g++)
-      EK_HIP(c, hipGraphLaunch(c->graph, c->stream));
+      EK_HIP(c, hipGraphLaunch(G.exec, c->stream));
   }
   for (int r = 0; r < rem; r++) {
     ek::StepCtl ctl{};

@@ -93,7 +93,11 @@ int ek_step(ek_ctx* ctx, double dt);
  * tEnd), advances the state and reduces the next invDt. On entry *t_io / *it_io hold the
  * current time and iteration; on exit they hold the values after the last iteration run,
  * and *dt_last the last dt. Stops early (without error) when t >= tEnd-1e-14 or
- * it >= nStepmax. Launches are replayed from a captured HIP graph. Synchronizes. */
+ * it >= nStepmax. Launches are replayed from a captured HIP graph (one per state buffer the
+ * run starts from, kept across calls). nsteps = 0 launches nothing and leaves *t_io, *it_io
+ * and *dt_last as they are; it only captures the graph for this start buffer, nStepmax and
+ * tEnd if it is not there yet, so that the capture stays out of a following timed run.
+ * Synchronizes (nsteps > 0). */
 int ek_run(ek_ctx* ctx, int nsteps, int nStepmax, double tEnd, double* t_io, int* it_io,
            double* dt_last);
 
