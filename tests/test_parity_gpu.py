@@ -107,6 +107,36 @@ def test_host_driven_step_matches_device_loop(oracle):
     assert rel_err(g, o) <= RTOL
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_run_in_pieces_and_capture_only_calls(oracle, dim):
+    # runs of odd and even lengths alternate the start buffer (one cached graph per buffer);
+    # nsteps = 0 only captures and changes nothing; the pieces equal one run and the oracle
+    if dim == 2:
+        ekp = make_params("Hydro_Muscl_2D", 40, 28, riemann="hllc", slope_type="2", cfl="0.8")
+        U0 = oracle.implode(oracle.from_ek(ekp))
+    else:
+        ekp = make_params("Hydro_Muscl_3D", 20, 18, 16, riemann="hllc", slope_type="2", cfl="0.5",
+                          bc=(3,) * 6)
+        U0 = oracle.blast(oracle.from_ek(ekp), radius=0.2, center_x=0.5, center_y=0.5, center_z=0.5,
+                          density_in=1.0, density_out=1.0, pressure_in=10.0, pressure_out=0.1)
+    g, o = run_pair(oracle, ekp, U0, 31)
+    ctx = native.Context(ekp)
+    try:
+        ctx.upload(U0)
+        t, it = 0.0, 0
+        for n in (0, 7, 0, 6, 12, 0, 1, 5):
+            t2, it2, _ = ctx.run(n, 31, 1e30, t, it)
+            if n == 0:
+                assert (t2, it2) == (t, it)
+            t, it = t2, it2
+        assert it == 31
+        G = ctx.download()
+    finally:
+        ctx.close()
+    assert np.array_equal(oracle.interior(oracle.from_ek(ekp), G), g)
+    assert rel_err(g, o) <= RTOL
+
+
 def test_tend_clamp_and_finish(oracle):
     # the device loop clamps the last dt to tEnd and stops (SolverBase::compute_dt/finished)
     ekp = make_params("Hydro_Muscl_2D", 32, 32, riemann="hllc", slope_type="2", cfl="0.8")
