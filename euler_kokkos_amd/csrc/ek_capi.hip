@@ -11,6 +11,46 @@
 #include <string>
 #include <vector>
 
+namespace ek {
+
+// FP64 -> FP32, round to nearest; this unit flushes no denormals, so the result is the
+// host's (float) conversion
+__global__ __launch_bounds__(256) void cast_f64_f32_kernel(const double* __restrict__ src, float* __restrict__ dst,
+                                                           long long n)
+{
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    dst[i] = (float)src[i];
+}
+
+hipError_t cast_f64_f32(const double* src, float* dst, long long n, hipStream_t s)
+{
+  if (n <= 0)
+    return hipSuccess;
+  const long long nb = std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(cast_f64_f32_kernel, dim3((unsigned)nb), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t upload_f64_as_f32(const double* host, float* dst, long long n, double* stage, long long stage_elems,
+                             hipStream_t s)
+{
+  // one staging buffer: each chunk's copy is queued behind the previous chunk's rounding
+  for (long long off = 0; off < n; off += stage_elems) {
+    const long long len = std::min(stage_elems, n - off);
+    hipError_t e = hipMemcpyAsync(stage, host + off, (size_t)len * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = cast_f64_f32(stage, dst + off, len, s);
+    if (e != hipSuccess)
+      return e;
+  }
+  return hipSuccess;
+}
+
+} // namespace ek
+
+// staging chunk of FP32 uploads (FP64 values)
+static constexpr long long kUploadChunk = 1LL << 22;
+
 struct ek_ctx {
   ek_params p{};
   std::string err;
@@ -24,6 +64,7 @@ struct ek_ctx {
   double* partial = nullptr;
   double* stage = nullptr;   // FP64 staging buffer for reference-layout transfers
   void* work = nullptr;      // 3D MHD ghost-padded work area
+  long long work_bytes = 0;
   long long stage_elems = 0;
   int nblocks_budget = 0;
   size_t elem = 8;
@@ -166,6 +207,7 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
       return fail(nullptr, EK_ENOMEM, "device allocation of the 3D MHD work area failed");
     }
     c->bytes += (long long)wb;
+    c->work_bytes = (long long)wb;
   }
   hipMemsetAsync(c->U[0], 0, state_bytes, c->stream);
   hipMemsetAsync(c->U[1], 0, state_bytes, c->stream);
@@ -241,11 +283,17 @@ extern "C" int ek_upload(ek_ctx* c, const double* host, int layout)
     if (p.precision == EK_FP64) {
       EK_HIP(c, hipMemcpyAsync(c->U[c->cur], host, n * 8, hipMemcpyHostToDevice, c->stream));
     } else {
-      std::vector<float> tmp(n);
-      for (long long i = 0; i < n; i++)
-        tmp[i] = (float)host[i];
-      EK_HIP(c, hipMemcpyAsync(c->U[c->cur], tmp.data(), n * 4, hipMemcpyHostToDevice, c->stream));
-      EK_HIP(c, hipStreamSynchronize(c->stream));
+      // FP64 values through a device staging buffer, rounded to FP32 on the device; a 3D
+      // MHD context stages in its work area, which holds nothing between steps
+      double* stage = (double*)c->work;
+      long long se = c->work_bytes / (long long)sizeof(double);
+      if (!stage) {
+        if (int r = ensure_stage(c, std::min(n, kUploadChunk)))
+          return r;
+        stage = c->stage;
+        se = c->stage_elems;
+      }
+      EK_HIP(c, ek::upload_f64_as_f32(host, (float*)c->U[c->cur], n, stage, se, c->stream));
     }
   } else if (layout == EK_LAYOUT_REFERENCE) {
     const long long n = ref_elems(p);
@@ -533,4 +581,21 @@ extern "C" int ek_last_run_time(const ek_ctx* c, double* ms, int* launches)
 extern "C" long long ek_device_bytes(const ek_ctx* c)
 {
   return c ? c->bytes + c->stage_elems * 8 : 0;
+}
+
+extern "C" int ek_host_alloc(size_t bytes, void** ptr)
+{
+  if (!ptr)
+    return EK_EINVAL;
+  *ptr = nullptr;
+  if (bytes == 0)
+    return EK_OK;
+  return hipHostMalloc(ptr, bytes, hipHostMallocDefault) == hipSuccess ? EK_OK : EK_ENOMEM;
+}
+
+extern "C" int ek_host_free(void* ptr)
+{
+  if (!ptr)
+    return EK_OK;
+  return hipHostFree(ptr) == hipSuccess ? EK_OK : EK_EHIP;
 }

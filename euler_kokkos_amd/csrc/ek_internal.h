@@ -13,6 +13,13 @@
 namespace ek {
 // Checks an ek_params against what this library provides (ek_capi.hip); 0 when valid.
 int validate_params(const ek_params* p, std::string& why);
+// FP32 copy of n FP64 values already on the device (round to nearest, denormals kept, as
+// the host's (float) conversion); ek_capi.hip.
+hipError_t cast_f64_f32(const double* src, float* dst, long long n, hipStream_t s);
+// n FP64 host values into the FP32 device array dst: chunks through the FP64 device buffer
+// stage of stage_elems values, rounded on the device; ek_capi.hip.
+hipError_t upload_f64_as_f32(const double* host, float* dst, long long n, double* stage, long long stage_elems,
+                             hipStream_t s);
 
 namespace fp64 {
 hipError_t step(int dim, int rs, int st, const double* Uin, double* Uout, const Geom<double>& geo,

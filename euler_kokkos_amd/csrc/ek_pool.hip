@@ -439,10 +439,10 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
       PHIP(p, hipMemcpyAsync(dst, host, (size_t)nelem * 8, hipMemcpyHostToDevice, p->stream));
       PHIP(p, hipStreamSynchronize(p->stream));
     } else {
-      std::vector<float> tmp((size_t)nelem);
-      for (long long i = 0; i < nelem; i++)
-        tmp[i] = (float)host[i];
-      PHIP(p, hipMemcpyAsync(dst, tmp.data(), (size_t)nelem * 4, hipMemcpyHostToDevice, p->stream));
+      // FP64 values through the device staging buffer, rounded to FP32 on the device
+      if ((rc = ensure_stage(p, std::min<long long>(nelem, 1LL << 22))))
+        return rc;
+      PHIP(p, ek::upload_f64_as_f32(host, (float*)dst, nelem, p->stage, p->stage_elems, p->stream));
       PHIP(p, hipStreamSynchronize(p->stream));
     }
   } else if (layout == EK_LAYOUT_REFERENCE) {

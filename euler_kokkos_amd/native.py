@@ -20,7 +20,8 @@ RIEMANN = {"approx": 0, "llf": 1, "hll": 2, "hllc": 3, "hlld": 4}
 
 # Exported symbols declared by include/ek_hip.h (checked by the CPU test-suite).
 EXPORTS = ("ek_create", "ek_destroy", "ek_last_error", "ek_upload", "ek_download", "ek_inv_dt",
-           "ek_step", "ek_run", "ek_budgets", "ek_sync", "ek_device_bytes", "ek_last_run_time")
+           "ek_step", "ek_run", "ek_budgets", "ek_sync", "ek_device_bytes", "ek_last_run_time",
+           "ek_host_alloc", "ek_host_free")
 
 
 class EkParams(C.Structure):
@@ -69,10 +70,48 @@ def lib() -> C.CDLL:
     L.ek_last_run_time.restype = C.c_int
     L.ek_device_bytes.argtypes = [C.c_void_p]
     L.ek_device_bytes.restype = C.c_longlong
-    for n in ("ek_upload", "ek_download", "ek_inv_dt", "ek_step", "ek_run", "ek_budgets", "ek_sync"):
+    L.ek_host_alloc.argtypes = [C.c_size_t, P(C.c_void_p)]
+    L.ek_host_free.argtypes = [C.c_void_p]
+    for n in ("ek_upload", "ek_download", "ek_inv_dt", "ek_step", "ek_run", "ek_budgets", "ek_sync",
+              "ek_host_alloc", "ek_host_free"):
         getattr(L, n).restype = C.c_int
     _LIB = L
     return L
+
+
+class _Pinned:
+    """Owner of one page-locked host block (ek_host_alloc); freed with its last array."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = C.c_void_p()
+        rc = lib().ek_host_alloc(C.c_size_t(nbytes), C.byref(self.ptr))
+        if rc != 0:
+            raise MemoryError(f"ek_host_alloc({nbytes}) failed ({rc})")
+
+    def __del__(self):
+        if self.ptr:
+            lib().ek_host_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def pinned_empty(shape, dtype=np.float64) -> np.ndarray:
+    """An uninitialised numpy array in page-locked host memory: uploads from it and
+    downloads into it run at the link's full speed. The memory is freed when the array and
+    every view of it are gone."""
+    dt = np.dtype(dtype)
+    count = int(np.prod(shape, dtype=np.int64))
+    nbytes = max(count * dt.itemsize, 1)
+    owner = _Pinned(nbytes)
+    buf = (C.c_char * nbytes).from_address(owner.ptr.value)
+    buf._owner = owner  # the ctypes buffer keeps the block alive; numpy keeps the buffer
+    return np.frombuffer(buf, dtype=dt, count=count).reshape(shape)
+
+
+def pinned_copy(a: np.ndarray) -> np.ndarray:
+    """A C-contiguous copy of `a` in page-locked host memory."""
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out
 
 
 class EkError(RuntimeError):

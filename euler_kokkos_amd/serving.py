@@ -85,9 +85,12 @@ def job_state(j: "Job") -> np.ndarray:
 
 
 def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh", "mhd_blast"), edge=(64, 192), steps=50,
-                   precision: int = native.EK_FP32, build_state: bool = True):
+                   precision: int = native.EK_FP32, build_state: bool = True, pinned: bool = True):
     """Seeded job stream: exponential inter-arrival times of mean 1/rate seconds, kind
-    uniform over `kinds`, edge uniform over the integers [edge[0], edge[1]]."""
+    uniform over `kinds`, edge uniform over the integers [edge[0], edge[1]]. With
+    build_state, each job's initial state is built now and, with `pinned`, held in
+    page-locked host memory, as a request's input buffer would be, so that its upload at
+    admission runs at the link's full speed."""
     rng = MT19937_64(seed)
     t = 0.0
     jobs = []
@@ -100,7 +103,7 @@ def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh", "mhd_
         j.B0 = 0.1 + 0.9 * rng.uniform()
         j.params = job_params(kind, n, precision)
         if build_state:
-            j.U0 = job_state(j)
+            j.U0 = native.pinned_copy(job_state(j)) if pinned else job_state(j)
         jobs.append(j)
     return jobs
 

@@ -2,8 +2,8 @@
  * ek_hip.h -- C ABI of the MI355X-native MUSCL-Hancock Godunov step (libek_hip.so).
  *
  * The library replaces the device side of euler_kokkos' MUSCL solvers -- hydro
- * (SolverHydroMuscl<2|3>, src/muscl/SolverHydroMuscl.{h,cpp}) and 2D MHD
- * (SolverMHDMuscl<2>, src/muscl/SolverMHDMuscl.{h,cpp}): the Kokkos functors of one
+ * (SolverHydroMuscl<2|3>, src/muscl/SolverHydroMuscl.{h,cpp}) and MHD
+ * (SolverMHDMuscl<2|3>, src/muscl/SolverMHDMuscl.{h,cpp}): the Kokkos functors of one
  * impl-0 godunov step, the ghost fill and the CFL reduction. A host solver keeps the
  * reference's Solver/ConfigMap interface and calls these entry points (see the C++ host
  * in euler_kokkos_amd/host/). Paths cited are relative to the euler_kokkos source tree.
@@ -14,6 +14,8 @@
  */
 #ifndef EK_HIP_H_
 #define EK_HIP_H_
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -117,6 +119,14 @@ int ek_last_run_time(const ek_ctx* ctx, double* ms, int* launches);
 
 /* Bytes of device memory held by the context. */
 long long ek_device_bytes(const ek_ctx* ctx);
+
+/* Page-locked host memory (hipHostMalloc) for states that are uploaded or downloaded:
+ * ek_upload / ek_download / ek_pool_admit / ek_pool_download copy from and to it at the
+ * link's full speed instead of through the runtime's bounce buffers. Returns EK_OK or
+ * EK_ENOMEM; free with ek_host_free. (No reference counterpart: the reference's host
+ * mirrors are Kokkos HostSpace views.) */
+int ek_host_alloc(size_t bytes, void** ptr);
+int ek_host_free(void* ptr);
 
 #ifdef __cplusplus
 }
