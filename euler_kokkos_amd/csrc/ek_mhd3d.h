@@ -3,8 +3,8 @@
 // One step is SolverMHDMuscl<3>::godunov_unsplit_impl, impl 0 (src/muscl/SolverMHDMuscl.cpp:
 // 462-512), as a pipeline of kernels over a ghost-padded SoA work area (x fastest, 3 ghost
 // layers each side, so padded index (i,j,k) is the reference's U(i,j,k)):
-//   pad     : interior state -> padded state, ghosts through the boundary map
-//   prims   : primitives with the right/top/front face fields (ConvertToPrimitivesFunctor3D_MHD)
+//   padprims: interior state -> padded state, ghosts through the boundary map, and the
+//             primitives with the right/top/front face fields (ConvertToPrimitivesFunctor3D_MHD)
 //   efield  : edge electric fields and transverse face-field slopes (ComputeElecFieldFunctor3D,
 //             ComputeMagSlopesFunctor3D)
 //   trace   : the half-step record of every cell (trace_unsplit_mhd_3d_simpler): centred
@@ -125,11 +125,14 @@ __device__ __forceinline__ bool step_dt(const StepCtl& c, double& dt)
   return true;
 }
 
-// pad: ghost values as SolverBase::make_boundaries fills them (X, then Y, then Z passes;
-// reflecting axes negate the normal momentum and the normal face field)
+// pad + prims in one pass. The padded state: ghost values as SolverBase::make_boundaries
+// fills them (X, then Y, then Z passes; reflecting axes negate the normal momentum and the
+// normal face field). On i < PX-1, j < PY-1, k < PZ-1 also the primitives
+// (ConvertToPrimitivesFunctor3D_MHD), whose right / top / front face fields are the padded
+// values of those neighbours, read from the interior state through the same map.
 template <class T>
-__global__ __launch_bounds__(256) void pad_kernel(const T* __restrict__ U, T* __restrict__ S, Work3 w, Geom<T> geo,
-                                                  StepCtl ctl)
+__global__ __launch_bounds__(256) void padprims_kernel(const T* __restrict__ U, T* __restrict__ S, Work3 w,
+                                                       Geom<T> geo, Gas<T> gas, StepCtl ctl)
 {
   double dt;
   if (!step_dt(ctl, dt))
@@ -141,39 +144,40 @@ __global__ __launch_bounds__(256) void pad_kernel(const T* __restrict__ U, T* __
   const int si = bc_map(i - 3, w.nx, geo.bc[0], geo.bc[1], fx);
   const int sj = bc_map(j - 3, w.ny, geo.bc[2], geo.bc[3], fy);
   const int sk = bc_map(k - 3, w.nz, geo.bc[4], geo.bc[5], fz);
+  const long long vs = geo.vstride;
   const long long o = ((long long)sk * w.ny + sj) * w.nx + si;
-  const long long po = pidx(w, i, j, k);
+  const long long po = pidx(w, i, j, k), PN = w.PN;
+  T u[NV];
 #pragma unroll
   for (int v = 0; v < NV; v++) {
-    T x = U[v * geo.vstride + o];
+    T x = U[v * vs + o];
     if ((fx && (v == IU || v == IA)) || (fy && (v == IV || v == IB)) || (fz && (v == IW || v == IC)))
       x = x * T(-1);
-    S[(A_UP + v) * w.PN + po] = x;
+    S[(A_UP + v) * PN + po] = x;
+    u[v] = x;
   }
-}
-
-// prims on i < PX-1, j < PY-1, k < PZ-1
-template <class T>
-__global__ __launch_bounds__(256) void prims_kernel(T* __restrict__ S, Work3 w, Gas<T> gas, StepCtl ctl)
-{
-  double dt;
-  if (!step_dt(ctl, dt))
+  if (i >= w.PX - 1 || j >= w.PY - 1 || k >= w.PZ - 1)
     return;
-  int i, j, k;
-  if (!cell_of<T>(w, i, j, k) || i >= w.PX - 1 || j >= w.PY - 1 || k >= w.PZ - 1)
-    return;
-  const long long PN = w.PN;
-  const unsigned pb = pidx(w, i, j, k) * (unsigned)sizeof(T);
-  T u[NV], q[NV];
+  // the padded IA of (i+1,j,k), IB of (i,j+1,k), IC of (i,j,k+1): only the reflection
+  // along the field's own axis negates it
+  bool gx, gy, gz;
+  const int si1 = bc_map(i - 2, w.nx, geo.bc[0], geo.bc[1], gx);
+  const int sj1 = bc_map(j - 2, w.ny, geo.bc[2], geo.bc[3], gy);
+  const int sk1 = bc_map(k - 2, w.nz, geo.bc[4], geo.bc[5], gz);
+  T mx = U[IA * vs + ((long long)sk * w.ny + sj) * w.nx + si1];
+  T my = U[IB * vs + ((long long)sk * w.ny + sj1) * w.nx + si];
+  T mz = U[IC * vs + ((long long)sk1 * w.ny + sj) * w.nx + si];
+  if (gx)
+    mx = mx * T(-1);
+  if (gy)
+    my = my * T(-1);
+  if (gz)
+    mz = mz * T(-1);
+  T q[NV];
+  constoprim<T>(gas, u, mx, my, mz, q);
 #pragma unroll
   for (int v = 0; v < NV; v++)
-    u[v] = ldb(S + (A_UP + v) * PN, pb);
-  constoprim<T>(gas, u, ldb(S + (A_UP + IA) * PN, pb + (unsigned)sizeof(T)),
-                ldb(S + (A_UP + IB) * PN, pb + (unsigned)w.PX * (unsigned)sizeof(T)),
-                ldb(S + (A_UP + IC) * PN, pb + (unsigned)(w.PX * w.PY) * (unsigned)sizeof(T)), q);
-#pragma unroll
-  for (int v = 0; v < NV; v++)
-    stb(S + (A_Q + v) * PN, pb, q[v]);
+    S[(A_Q + v) * PN + po] = q[v];
 }
 
 // edge electric fields and transverse face-field slopes on 1 <= i,j,k < size-1

@@ -12,8 +12,7 @@ static hipError_t launch3(const float* Uin, float* Uout, float* S, const mhd3::W
   const unsigned nb = (unsigned)((w.PN + 255) / 256);
   const long long nu = (geo.vstride + 255) / 256;
   const unsigned ni = (unsigned)(nu < mhd3::MHD3_UPDATE_BLOCKS ? nu : mhd3::MHD3_UPDATE_BLOCKS);
-  hipLaunchKernelGGL(mhd3::pad_kernel<float>, dim3(nb), dim3(256), 0, s, Uin, S, w, geo, ctl);
-  hipLaunchKernelGGL(mhd3::prims_kernel<float>, dim3(nb), dim3(256), 0, s, S, w, gas, ctl);
+  hipLaunchKernelGGL(mhd3::padprims_kernel<float>, dim3(nb), dim3(256), 0, s, Uin, S, w, geo, gas, ctl);
   hipLaunchKernelGGL((mhd3::efield_kernel<float, ST>), dim3(nb), dim3(256), 0, s, S, w, ctl);
   hipLaunchKernelGGL((mhd3::trace_kernel<float, ST>), dim3(nb), dim3(256), 0, s, S, w, geo, gas, ctl);
   const unsigned nfe = 32u * ((nb + 7u) / 8u); // 4 tasks per chunk, chunks in groups of 8
