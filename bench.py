@@ -98,7 +98,7 @@ def cpu_baseline(n, seconds):
 
 def mhd3d_c5(n, steps, with_cpu, cpu_seconds):
     """3D ideal-MHD blast (the serving stream's MHD job kind) on n^3, periodic, HLLD + CT,
-    gamma 1.6666667, CFL 0.8, MC slopes, field B0 = 0.5 along (1,1,1): the six-kernel step
+    gamma 1.6666667, CFL 0.8, MC slopes, field B0 = 0.5 along (1,1,1): the five-kernel step
     pipeline in FP64 and FP32, the discrete max |div B| of the FP64 result, and the oracle
     on the host cores."""
     from euler_kokkos_amd import native, problems
