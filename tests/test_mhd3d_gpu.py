@@ -84,6 +84,27 @@ def test_mhd3d_fp64_parity(oracle, rs, st, bc):
     assert np.array_equal(g, o), rel(g, o)
 
 
+def test_mhd3d_oblique_alfven_wave_bitwise(oracle):
+    # the oracle's second-order oblique Alfven wave (test_oracle_mhd3d.py), run to the time it
+    # returns to its initial state, on the GPU: bitwise the oracle's result
+    from test_oracle_mhd3d import oblique_alfven
+    ekp = m3(24, 24, 24)
+    op, R0, T = oblique_alfven(oracle, 24)
+    assert (op.nx, op.gamma0, op.cfl) == (oracle.from_ek(ekp).nx, oracle.from_ek(ekp).gamma0, ekp.cfl)
+    R = R0.copy()
+    t_o, it_o, _ = oracle.mhd3d_run(op, R, 100000, T)
+    ctx = native.Context(ekp)
+    try:
+        ctx.upload(R0)
+        t, it, _ = ctx.run(100000, 100000, T)
+        G = ctx.download()
+    finally:
+        ctx.close()
+    assert (t, it) == (t_o, it_o)
+    print(f"case=mhd3d oblique Alfven 24^3 to T={T:.6f} ({it} steps)", flush=True)
+    assert np.array_equal(G[3:-3, 3:-3, 3:-3], R[3:-3, 3:-3, 3:-3])
+
+
 def test_mhd3d_blast_host_driven(oracle):
     ekp = m3(20, 20, 20)
     U = problems.mhd_blast3d(ekp, B0=0.6)
