@@ -59,6 +59,18 @@ def run_pair(oracle, ekp, R0, nsteps, tEnd=1e30, use_step=False):
     return oracle.interior(op, G), oracle.interior(op, R)
 
 
+def test_oblique_alfven_wave_bitwise(oracle):
+    # the oracle's second-order oblique Alfven wave (test_oracle_mhd.py) to the time it returns
+    # to its initial state: the GPU result is bitwise the oracle's
+    from test_oracle_mhd import oblique_alfven_2d
+    ekp = mparams(64, 64)
+    op, R0, T = oblique_alfven_2d(oracle, 64)
+    assert (op.nx, op.ny, op.gamma0) == (ekp.nx, ekp.ny, oracle.from_ek(ekp).gamma0)
+    g, o = run_pair(oracle, ekp, R0, 100000, tEnd=T)
+    print(f"case=mhd2d oblique Alfven 64^2 to T={T:.6f}", flush=True)
+    assert np.array_equal(g, o), rel_err(g, o)
+
+
 @pytest.mark.parametrize("rs,st", [("hlld", "2"), ("hlld", "1"), ("hll", "2"), ("llf", "1")])
 def test_orszag_tang_fp64_parity(oracle, rs, st):
     ekp = mparams(64, 48, riemann=rs, slope_type=st)  # ragged against the 14x14 tiles
