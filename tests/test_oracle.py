@@ -199,6 +199,40 @@ def test_trace_linear_advection_predictor(oracle):
     assert ylo[0] == pytest.approx(r - u * dr * dt / p.dy / 2, rel=1e-15)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_trace_predictor_is_the_primitive_euler_rhs(oracle, dim):
+    # On a cubic grid the Hancock half step of trace_unsplit_{2d,3d}_along_dir is
+    # q -/+ dq_d / 2 + (dt / 2) dq/dt with dq/dt the primitive Euler equations evaluated on the
+    # slopes (dq_d = dx * d q / d x_d):
+    #   rho_t = -u.grad rho - rho div u,  u_t = -u.grad u - grad p / rho,
+    #   p_t = -u.grad p - gamma p div u.
+    # Random states, slopes and time steps; every advective and cross-direction term counts.
+    solver = "Hydro_Muscl_2D" if dim == 2 else "Hydro_Muscl_3D"
+    p = P(oracle, solver, nx=16, ny=16, **({"nz": 16} if dim == 3 else {}))
+    assert p.dx == p.dy and (dim == 2 or p.dz == p.dx)
+    g = p.gamma0
+    rng = np.random.default_rng(11 + dim)
+    nv = 2 + dim
+    for _ in range(20):
+        q = np.array([rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0), *rng.uniform(-1, 1, dim)])
+        dqs = [rng.uniform(-0.1, 0.1, nv) for _ in range(dim)] + [np.zeros(nv)] * (3 - dim)
+        dt = rng.uniform(0.001, 0.01)
+        r, pr, vel = q[0], q[1], q[2:]
+        grad = np.array([d / p.dx for d in dqs[:dim]])          # grad[d][var]
+        div = sum(grad[d][2 + d] for d in range(dim))
+        adv = sum(vel[d] * grad[d] for d in range(dim))          # u.grad of every variable
+        qt = np.empty(nv)
+        qt[0] = -adv[0] - r * div
+        qt[1] = -adv[1] - g * pr * div
+        for c in range(dim):
+            qt[2 + c] = -adv[2 + c] - grad[c][1] / r
+        for face in range(2 * dim):
+            d, side = face // 2, (1 if face % 2 else -1)
+            want = q + side * 0.5 * dqs[d] + 0.5 * dt * qt
+            got = trace(oracle, p, q, dqs[0], dqs[1], dqs[2], dt, face)
+            np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-15)
+
+
 # ------------------------------------------------------------------ boundaries (a9)
 
 def np_ghost_fill(U, gw, bc, dim, nvar_normal=(2, 3, 4)):
