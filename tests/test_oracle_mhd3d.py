@@ -219,3 +219,83 @@ def test_oblique_alfven_wave_second_order(oracle):
     for c in (IU, IV, IW, IA, IB, IC):
         assert errs[1][c] < 0.05, (c, errs)
         assert errs[0][c] / errs[1][c] > 3.0, (c, errs)
+
+
+def _trace3(oracle, p, q, dq, bf, dbf, E, dt):
+    D = C.POINTER(C.c_double)
+    qm, qp, qe = np.zeros((3, 8)), np.zeros((3, 8)), np.zeros((4, 3, 8))
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (q, dq, bf, dbf, E)]
+    L = oracle.lib()
+    L.eko_mhd3d_trace.restype = None
+    L.eko_mhd3d_trace(C.byref(p), *[a.ctypes.data_as(D) for a in args], C.c_double(dt / p.dx),
+                      C.c_double(dt / p.dy), C.c_double(dt / p.dz), qm.ctypes.data_as(D), qp.ctypes.data_as(D),
+                      qe.ctypes.data_as(D))
+    return qm, qp, qe
+
+
+def test_trace3d_is_the_primitive_mhd_half_step(oracle):
+    # trace_unsplit_mhd_3d_simpler (MHDBaseFunctor3D.h:690-1110) on random data, against an
+    # independent statement: the centre moves by dt/2 times the primitive ideal-MHD time
+    # derivative evaluated on the slopes (d q / d x_k = dq_k / dx_k; the normal derivative of
+    # each field component from its two face values), each face field by dt/2 (curl E) from
+    # its four edge fields (E = v x B: dB/dt = curl E); a face state is the centre plus or
+    # minus half the slope, its normal field the updated face value; an edge state adds the
+    # two transverse half slopes, and its two transverse fields are the updated face values
+    # on its sides plus or minus their own transverse half slope.
+    p = P3(oracle, 12, 10, 8, xmax="1.2", zmax="0.9")
+    dxs = (p.dx, p.dy, p.dz)
+    g = p.gamma0
+    rng = np.random.default_rng(21)
+    for _ in range(20):
+        q = np.array([rng.uniform(0.5, 2), rng.uniform(0.5, 2), *rng.uniform(-1, 1, 3), *rng.uniform(-1, 1, 3)])
+        dq = rng.uniform(-0.05, 0.05, (3, 8))
+        bf = q[[IA, IA, IB, IB, IC, IC]] + rng.uniform(-0.05, 0.05, 6)   # AL AR BL BR CL CR
+        dbf = rng.uniform(-0.05, 0.05, 12)
+        E = rng.uniform(-0.5, 0.5, (3, 2, 2))
+        dt = rng.uniform(0.001, 0.01)
+        qm, qp, qe = _trace3(oracle, p, q, dq, bf, dbf, E, dt)
+
+        vel, Bc = q[IU:IW + 1], q[IA:IC + 1]
+        grad = dq / np.array(dxs)[:, None]
+        dBn = [(bf[1] - bf[0]) / p.dx, (bf[3] - bf[2]) / p.dy, (bf[5] - bf[4]) / p.dz]
+        divu = sum(grad[k][IU + k] for k in range(3))
+        t = np.zeros(8)
+        t[ID] = -sum(vel[k] * grad[k][ID] for k in range(3)) - q[ID] * divu
+        t[IP] = -sum(vel[k] * grad[k][IP] for k in range(3)) - g * q[IP] * divu
+        for i in range(3):
+            t[IU + i] = (-sum(vel[k] * grad[k][IU + i] for k in range(3)) - grad[i][IP] / q[ID]
+                         - sum(Bc[j] * grad[i][IA + j] for j in range(3) if j != i) / q[ID]
+                         + sum(Bc[k] * grad[k][IA + i] for k in range(3) if k != i) / q[ID])
+            t[IA + i] = sum(vel[i] * dBn[k] + Bc[k] * grad[k][IU + i] - vel[k] * grad[k][IA + i]
+                            - Bc[i] * grad[k][IU + k] for k in range(3) if k != i)
+        qs = q + 0.5 * dt * t
+        # face fields: curl E over the face's four edges; E[c][a][b]: component c at the edges
+        # on sides (a, b) of the two directions other than c, in increasing axis order
+        curl = {
+            (0, 0): (E[2][0][1] - E[2][0][0]) / p.dy - (E[1][0][1] - E[1][0][0]) / p.dz,
+            (0, 1): (E[2][1][1] - E[2][1][0]) / p.dy - (E[1][1][1] - E[1][1][0]) / p.dz,
+            (1, 0): (E[0][0][1] - E[0][0][0]) / p.dz - (E[2][1][0] - E[2][0][0]) / p.dx,
+            (1, 1): (E[0][1][1] - E[0][1][0]) / p.dz - (E[2][1][1] - E[2][0][1]) / p.dx,
+            (2, 0): (E[1][1][0] - E[1][0][0]) / p.dx - (E[0][1][0] - E[0][0][0]) / p.dy,
+            (2, 1): (E[1][1][1] - E[1][0][1]) / p.dx - (E[0][1][1] - E[0][0][1]) / p.dy,
+        }
+        face = {(c, s): bf[2 * c + s] + 0.5 * dt * curl[(c, s)] for c in range(3) for s in range(2)}
+        # dbf: for each field component c and side s, its half slope along the two other axes
+        dbi = {(0, 0): (None, 0, 1), (1, 0): (2, None, 3), (2, 0): (4, 5, None),
+               (0, 1): (None, 6, 7), (1, 1): (8, None, 9), (2, 1): (10, 11, None)}
+        hyd = [ID, IP, IU, IV, IW]
+        for d in range(3):
+            for side, out in ((1, qm[d]), (0, qp[d])):
+                sg = 1.0 if side else -1.0
+                want = qs + sg * 0.5 * dq[d]
+                want[IA + d] = face[(d, side)]
+                np.testing.assert_allclose(out, want, rtol=1e-13, atol=1e-15)
+            a, b = [k for k in range(3) if k != d]
+            for e, (s1, s2) in enumerate(((1, 1), (1, 0), (0, 1), (0, 0))):
+                g1, g2 = (1.0 if s1 else -1.0), (1.0 if s2 else -1.0)
+                want = qs + g1 * 0.5 * dq[a] + g2 * 0.5 * dq[b]
+                want[IA + a] = face[(a, s1)] + g2 * 0.5 * dbf[dbi[(a, s1)][b]]
+                want[IA + b] = face[(b, s2)] + g1 * 0.5 * dbf[dbi[(b, s2)][a]]
+                np.testing.assert_allclose(qe[e][d][hyd + [IA + d]], want[hyd + [IA + d]], rtol=1e-13, atol=1e-15)
+                np.testing.assert_allclose(qe[e][d][[IA + a, IA + b]], want[[IA + a, IA + b]], rtol=1e-13,
+                                           atol=1e-15)
