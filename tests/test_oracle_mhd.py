@@ -138,6 +138,29 @@ def test_hlld_resolves_stationary_contact(oracle):
     assert abs(fh[ID]) > 1e-3  # HLL smears it (shows the test has teeth)
 
 
+@pytest.mark.parametrize("theta", [0.7, 2.0, math.pi])
+def test_hlld_resolves_stationary_rotational_discontinuity(oracle, theta):
+    # An isolated Alfven (rotational) discontinuity is the other exact HLLD solution
+    # (Miyoshi & Kusano 2005): rho, p, u and Bx continuous, the tangential field rotated by
+    # theta and the tangential velocity jumping by -[B_t] / sqrt(rho), with u = -Bx / sqrt(rho)
+    # so that the wave is at rest. Both physical fluxes agree, and the HLLD flux is that flux.
+    p = P(oracle)
+    r, pr, bx = 1.3, 0.9, 0.8
+    u = -bx / math.sqrt(r)
+    btl, vtl = np.array([0.5, 0.2]), np.array([0.2, -0.1])
+    c, s = math.cos(theta), math.sin(theta)
+    btr = np.array([c * btl[0] - s * btl[1], s * btl[0] + c * btl[1]])
+    vtr = vtl - (btr - btl) / math.sqrt(r)
+    ql = np.array([r, pr, u, *vtl, bx, *btl])
+    qr = np.array([r, pr, u, *vtr, bx, *btr])
+    fl = mhd_physical_flux(ql, p.gamma0)
+    np.testing.assert_allclose(mhd_physical_flux(qr, p.gamma0), fl, rtol=0, atol=2e-15)
+    f = oracle.riemann_mhd(p, ql.copy(), qr.copy())
+    np.testing.assert_allclose(f, fl, rtol=0, atol=4e-15)
+    fh = oracle.riemann_mhd(P(oracle, riemann="hll"), ql.copy(), qr.copy())
+    assert np.abs(fh - fl).max() > 0.1  # HLL smears it (shows the test has teeth)
+
+
 # ------------------------------------------------------------------ 2D EMF solver
 
 def test_emfz_uniform_state_is_v_cross_b(oracle):
