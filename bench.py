@@ -365,7 +365,8 @@ def main() -> int:
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build has no multi-GPU "
-              "path; see DESIGN.md for how the grid would shard.", file=sys.stderr)
+              "path (a grid would shard into one subdomain per GPU with RCCL halo exchanges, "
+              "and batched jobs by job).", file=sys.stderr)
         return 2
     if a.copy_calibration:
         copy_calibration()
