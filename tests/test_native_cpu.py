@@ -42,3 +42,33 @@ def test_create_rejects_unsupported(bad):
     rc = native.lib().ek_create(C.byref(p), C.byref(h))
     assert rc == 1
     assert native.lib().ek_last_error(None)
+
+
+@pytest.mark.parametrize("case", ["mhd3d_too_large", "cIso", "mhd_slope", "bc", "tiny", "precision", "dim"])
+def test_create_rejects_invalid_before_allocating(case):
+    # every check runs on the host, before any device call (so it runs here without a GPU)
+    if case in ("mhd3d_too_large", "cIso", "mhd_slope"):
+        n = 1200 if case == "mhd3d_too_large" else 16
+        p = make_params("MHD_Muscl_3D", n, n, n, riemann="hlld", slope_type="2")
+    else:
+        p = make_params("Hydro_Muscl_3D", 16, 16, 16)
+    want = {"mhd3d_too_large": "4 GiB", "cIso": "isothermal", "mhd_slope": "slope_type",
+            "bc": "boundary", "tiny": "ghostWidth", "precision": "precision", "dim": "dim"}[case]
+    if case == "cIso":
+        p.cIso = 0.5
+    elif case == "mhd_slope":
+        p.slope_type = 0.0
+    elif case == "bc":
+        p.bc[3] = 4
+    elif case == "tiny":
+        p.ny = 1
+    elif case == "precision":
+        p.precision = 7
+    elif case == "dim":
+        p.dim = 4
+    h = C.c_void_p()
+    rc = native.lib().ek_create(C.byref(p), C.byref(h))
+    assert rc == 1 and not h.value
+    msg = native.lib().ek_last_error(None).decode()
+    assert want in msg, msg
+
