@@ -179,7 +179,7 @@ extern "C" int ek_create(const ek_params* params, ek_ctx** out)
   ek::fill_geom<double>(c->p, c->g64, c->gas64);
   ek::fill_geom<float>(c->p, c->g32, c->gas32);
   c->ncell = c->g64.vstride;
-  c->cz = params->dim == 3 ? (params->nz < 32 ? params->nz : 32) : 1;
+  c->cz = params->dim == 3 ? ek::hydro3d_z_chunk(params->nx, params->ny, params->nz) : 1;
   hipError_t e;
   if ((e = hipGetDevice(&c->device)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {

@@ -31,6 +31,23 @@
 
 namespace ek {
 
+// Column tile of the single-grid 3D step: 16 x 8 cells, 256 threads.
+constexpr int T3X = 16, T3Y = 8, N3T = 256;
+
+// z-chunk length of the single-grid 3D step for an nx x ny x nz grid: the longest of 32,
+// 24, 16, 12, 8 that still gives 2048 blocks (8 per CU, enough to fill the 256 CUs with a
+// short tail), else 4. Each chunk re-traces two planes it does not update, so the longest
+// chunk that fills the GPU is fastest; on small grids shorter chunks are what fills it
+// (64^3: 0.134 ms per step at 32, 0.037 at 4; 192^3: 0.551 at 32, 0.532 at 24; 256^3 keeps 32).
+inline int hydro3d_z_chunk(int nx, int ny, int nz)
+{
+  const long long tiles = (long long)((nx + T3X - 1) / T3X) * ((ny + T3Y - 1) / T3Y);
+  for (const int cz : { 32, 24, 16, 12, 8 })
+    if (tiles * ((nz + cz - 1) / cz) >= 2048)
+      return cz < nz ? cz : nz;
+  return 4 < nz ? 4 : nz;
+}
+
 template <int TX, int TY, int NT>
 struct Col3 {
   static constexpr int QX = TX + 4, QY = TY + 4, QN = QX * QY;

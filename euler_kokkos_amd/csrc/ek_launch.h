@@ -8,10 +8,10 @@
 
 namespace ek {
 
-// Tile shapes. 2D: 16x16 cells per block (256 threads). 3D: 16x8 column tiles marching
-// over z-chunks (256 threads: one per primitive column of the tile and its 2-cell halo).
+// Tile shapes. 2D: 16x16 cells per block (256 threads). 3D: T3X x T3Y column tiles marching
+// over z-chunks (ek_hydro3d.h; N3T threads: one per primitive column of the tile and its
+// 2-cell halo).
 constexpr int T2X = 16, T2Y = 16, N2T = 256;
-constexpr int T3X = 16, T3Y = 8, N3T = 256;
 
 template <class T>
 using Step2Fn = void (*)(const T*, T*, Geom<T>, Gas<T>, StepCtl);
