@@ -80,6 +80,18 @@ def test_hydro3d_blast_fp64_parity(oracle, rs):
     assert rel_err(g, o) <= RTOL
 
 
+@pytest.mark.parametrize("shape,cz", [((256, 128, 60), 8), ((192, 192, 100), 12)])
+def test_hydro3d_z_chunk_lengths_fp64_parity(oracle, shape, cz):
+    # grids whose z-chunk length (chosen from the grid size) is 8 or 12 with a partial last
+    # chunk: the chunking changes where planes are traced, never a result
+    ekp = make_params("Hydro_Muscl_3D", *shape, riemann="hllc", slope_type="2", cfl="0.5", bc=(3, 3, 1, 1, 2, 2))
+    U0 = oracle.blast(oracle.from_ek(ekp), radius=0.2, center_x=0.45, center_y=0.55, center_z=0.5,
+                      density_in=1.0, density_out=1.0, pressure_in=10.0, pressure_out=0.1)
+    g, o = run_pair(oracle, ekp, U0, 3)
+    print(f"case=hydro3d {shape} (z-chunk {cz}): max|diff|={np.max(np.abs(g - o)):.3e}", flush=True)
+    assert np.array_equal(g, o)
+
+
 def test_hydro3d_ragged_mixed_bc_fp64_parity(oracle):
     # sizes that are not multiples of the tile, every boundary type, non-cubic cells
     ekp = make_params("Hydro_Muscl_3D", 37, 21, 19, xmax="1.3", riemann="hllc", slope_type="2",
