@@ -235,6 +235,79 @@ def test_trace_linear_data_zero_dt(oracle):
         np.testing.assert_allclose(s[IB], 0.5 + sx * gx[IB] / 2, rtol=1e-14)
 
 
+def test_trace2d_is_the_primitive_mhd_half_step(oracle):
+    # trace_unsplit_mhd_2d (MHDBaseFunctor2D.h:774-1109) on random bilinear data, against an
+    # independent statement. Cell data q0 + gx x + gy y + gxy x y (cell units, centre cell at
+    # the origin): the MC slopes of the centre cell are exactly gx and gy, and the average of
+    # the four cells around a corner is the bilinear value there. Face fields
+    # Bx(x, y) = A0 + Ax x + Ay y + Axy x y on the x faces and By likewise on the y faces: the
+    # field at a corner is the value there, a face field's transverse MC slope is the exact
+    # centred difference (different on opposite faces when Axy, Bxy != 0). Then the centre
+    # moves by dt/2 times the primitive ideal-MHD time derivative with d/dz = 0 (normal field
+    # derivatives from the face pairs), each face field by dt/2 curl E with E = v x B at the
+    # corners, face states are the centre -+ half the slope with the updated normal field,
+    # and an edge state adds both half slopes, its fields being the updated faces on its sides
+    # moved half a cell along by their own transverse slope.
+    p = P(oracle, 12, 10, xmax="1.2")
+    g = p.gamma0
+    dxs = (p.dx, p.dy)
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        q0 = np.array([rng.uniform(0.8, 2), rng.uniform(0.8, 2), *rng.uniform(-1, 1, 3), *rng.uniform(-1, 1, 3)])
+        gx, gy, gxy = (rng.uniform(-0.04, 0.04, 8) for _ in range(3))
+        fa, fb = rng.uniform(-0.05, 0.05, 4), rng.uniform(-0.05, 0.05, 4)
+        fa[0], fb[0] = rng.uniform(-1, 1), rng.uniform(-1, 1)
+        cell = lambda x, y: q0 + gx * x + gy * y + gxy * x * y
+        Af = lambda x, y: fa[0] + fa[1] * x + fa[2] * y + fa[3] * x * y
+        Bf = lambda x, y: fb[0] + fb[1] * x + fb[2] * y + fb[3] * x * y
+        qnb = np.array([[cell(i - 1, j - 1) for j in range(3)] for i in range(3)])
+        bfnb = np.zeros((4, 4, 3))
+        for i in range(4):
+            for j in range(4):
+                bfnb[i, j] = [Af(i - 1.5, j - 1), Bf(i - 1, j - 1.5), 0.0]
+        dt = rng.uniform(0.001, 0.01)
+        qm, qp, qe = oracle.mhd_trace_2d(p, qnb, bfnb, dt / p.dx, dt / p.dy)
+
+        q = q0
+        vel, Bc = q[IU:IW + 1], q[IA:IC + 1]
+        grad = np.array([gx / dxs[0], gy / dxs[1], np.zeros(8)])
+        dBn = [(Af(0.5, 0) - Af(-0.5, 0)) / p.dx, (Bf(0, 0.5) - Bf(0, -0.5)) / p.dy, 0.0]
+        divu = grad[0][IU] + grad[1][IV]
+        t = np.zeros(8)
+        t[ID] = -sum(vel[k] * grad[k][ID] for k in range(3)) - q[ID] * divu
+        t[IP] = -sum(vel[k] * grad[k][IP] for k in range(3)) - g * q[IP] * divu
+        for i in range(3):
+            t[IU + i] = (-sum(vel[k] * grad[k][IU + i] for k in range(3)) - grad[i][IP] / q[ID]
+                         - sum(Bc[j] * grad[i][IA + j] for j in range(3) if j != i) / q[ID]
+                         + sum(Bc[k] * grad[k][IA + i] for k in range(3) if k != i) / q[ID])
+            t[IA + i] = sum(vel[i] * dBn[k] + Bc[k] * grad[k][IU + i] - vel[k] * grad[k][IA + i]
+                            - Bc[i] * grad[k][IU + k] for k in range(3) if k != i)
+        qs = q + 0.5 * dt * t
+
+        def Ez(x, y):
+            c = cell(x, y)
+            return c[IU] * Bf(x, y) - c[IV] * Af(x, y)
+
+        # dBx/dt = dEz/dy on the x faces (x = -+1/2), dBy/dt = -dEz/dx on the y faces
+        AL = Af(-0.5, 0) + 0.5 * dt * (Ez(-0.5, 0.5) - Ez(-0.5, -0.5)) / p.dy
+        AR = Af(0.5, 0) + 0.5 * dt * (Ez(0.5, 0.5) - Ez(0.5, -0.5)) / p.dy
+        BL = Bf(0, -0.5) - 0.5 * dt * (Ez(0.5, -0.5) - Ez(-0.5, -0.5)) / p.dx
+        BR = Bf(0, 0.5) - 0.5 * dt * (Ez(0.5, 0.5) - Ez(-0.5, 0.5)) / p.dx
+        for side, out in ((1, qm), (0, qp)):
+            sg = 1.0 if side else -1.0
+            for d, (nf, val) in enumerate(((IA, AR if side else AL), (IB, BR if side else BL))):
+                want = qs + sg * 0.5 * (gx, gy)[d]
+                want[nf] = val
+                np.testing.assert_allclose(out[d], want, rtol=1e-13, atol=1e-15)
+        RT, RB, LT, LB = qe
+        for s, sx, sy in ((RT, 1, 1), (RB, 1, -1), (LT, -1, 1), (LB, -1, -1)):
+            want = qs + sx * 0.5 * gx + sy * 0.5 * gy
+            xf, yf = 0.5 * sx, 0.5 * sy
+            want[IA] = (AR if sx > 0 else AL) + sy * 0.5 * (fa[2] + fa[3] * xf)
+            want[IB] = (BR if sy > 0 else BL) + sx * 0.5 * (fb[1] + fb[3] * yf)
+            np.testing.assert_allclose(s, want, rtol=1e-13, atol=1e-15)
+
+
 def test_trace_advection_predictor(oracle):
     # uniform velocity and pressure, no field: density is advected by half a step
     p = P(oracle)
