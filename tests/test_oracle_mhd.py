@@ -161,6 +161,93 @@ def test_hlld_resolves_stationary_rotational_discontinuity(oracle, theta):
     assert np.abs(fh - fl).max() > 0.1  # HLL smears it (shows the test has teeth)
 
 
+def hlld_jump_form(ql, qr, g):
+    """HLLD flux as published by Miyoshi & Kusano (J. Comput. Phys. 208, 2005, eqs. 38-66) in
+    its Rankine-Hugoniot form: each fan state is built in conservative variables and each
+    flux is the neighbouring one plus the wave speed times the state jump
+    (F*L = FL + SL (U*L - UL), F**L = F*L + S*L (U**L - U*L), mirrored on the right), with
+    the normal field the average of the two sides' and the Davis wave-speed bounds
+    SL = min(uL, uR) - max(cfL, cfR), SR = max(uL, uR) + max(cfL, cfR).
+    Returns (flux, region index 0..5 from left to right)."""
+    bx = 0.5 * (ql[IA] + qr[IA])
+
+    def prim(q):
+        r, p, u, v, w, _, b, c = q
+        return r, p, u, np.array([v, w]), np.array([b, c])
+
+    def cons(r, u, vt, bt, e):
+        return np.array([r, e, r * u, r * vt[0], r * vt[1], bx, bt[0], bt[1]])
+
+    def fast(r, p, bt):
+        c2, b2 = g * p / r, (bx * bx + bt @ bt) / r
+        return math.sqrt(0.5 * (c2 + b2 + math.sqrt((c2 + b2) ** 2 - 4 * c2 * bx * bx / r)))
+
+    sides = []
+    for q in (ql, qr):
+        r, p, u, vt, bt = prim(q)
+        pt = p + 0.5 * (bx * bx + bt @ bt)
+        e = p / (g - 1) + 0.5 * r * (u * u + vt @ vt) + 0.5 * (bx * bx + bt @ bt)
+        vb = u * bx + vt @ bt
+        F = np.array([r * u, (e + pt) * u - bx * vb, r * u * u + pt - bx * bx, r * u * vt[0] - bx * bt[0],
+                      r * u * vt[1] - bx * bt[1], 0.0, bt[0] * u - bx * vt[0], bt[1] * u - bx * vt[1]])
+        sides.append(dict(r=r, u=u, vt=vt, bt=bt, pt=pt, e=e, vb=vb, F=F, U=cons(r, u, vt, bt, e), cf=fast(r, p, bt)))
+    L, R = sides
+    cmax = max(L["cf"], R["cf"])
+    SL, SR = min(L["u"], R["u"]) - cmax, max(L["u"], R["u"]) + cmax
+    mL, mR = (SL - L["u"]) * L["r"], (SR - R["u"]) * R["r"]
+    SM = (mR * R["u"] - mL * L["u"] - R["pt"] + L["pt"]) / (mR - mL)
+    pts = (mR * L["pt"] - mL * R["pt"] + mL * mR * (R["u"] - L["u"])) / (mR - mL)
+    star = []
+    for s, S in ((L, SL), (R, SR)):
+        rs = s["r"] * (S - s["u"]) / (S - SM)
+        den = s["r"] * (S - s["u"]) * (S - SM) - bx * bx
+        vts = s["vt"] - bx * s["bt"] * (SM - s["u"]) / den
+        bts = s["bt"] * (s["r"] * (S - s["u"]) ** 2 - bx * bx) / den
+        vbs = SM * bx + vts @ bts
+        es = ((S - s["u"]) * s["e"] - s["pt"] * s["u"] + pts * SM + bx * (s["vb"] - vbs)) / (S - SM)
+        star.append(dict(r=rs, vt=vts, bt=bts, vb=vbs, e=es, U=cons(rs, SM, vts, bts, es), den=den))
+    sL, sR = star
+    sg = 1.0 if bx >= 0 else -1.0
+    qL, qR = math.sqrt(sL["r"]), math.sqrt(sR["r"])
+    vss = (qL * sL["vt"] + qR * sR["vt"] + sg * (sR["bt"] - sL["bt"])) / (qL + qR)
+    bss = (qL * sR["bt"] + qR * sL["bt"] + sg * qL * qR * (sR["vt"] - sL["vt"])) / (qL + qR)
+    vbss = SM * bx + vss @ bss
+    UssL = cons(sL["r"], SM, vss, bss, sL["e"] - sg * qL * (sL["vb"] - vbss))
+    UssR = cons(sR["r"], SM, vss, bss, sR["e"] + sg * qR * (sR["vb"] - vbss))
+    SaL, SaR = SM - abs(bx) / qL, SM + abs(bx) / qR
+    FsL = L["F"] + SL * (sL["U"] - L["U"])
+    FsR = R["F"] + SR * (sR["U"] - R["U"])
+    fluxes = [L["F"], FsL, FsL + SaL * (UssL - sL["U"]), FsR + SaR * (UssR - sR["U"]), FsR, R["F"]]
+    region = int(np.searchsorted([SL, SaL, SM, SaR, SR], 0.0))
+    scale = max(np.abs(L["F"]).max(), np.abs(R["F"]).max(), abs(SL) * np.abs(L["U"]).max(),
+                abs(SR) * np.abs(R["U"]).max())
+    return fluxes[region], region, scale, (sL["den"], sR["den"])
+
+
+def test_hlld_is_the_published_jump_form(oracle):
+    # riemann_hlld (RiemannSolvers_MHD.h:130-364) forms the sampled fan state and evaluates a
+    # flux from it; the published solver states each fan flux through the jump conditions.
+    # The two agree in exact arithmetic; on random states, shifted so that every one of the
+    # six fan regions is sampled, they agree to rounding, for both signs of the normal field.
+    p = P(oracle)
+    rng = np.random.default_rng(8)
+    hits = np.zeros(6, int)
+    for _ in range(400):
+        ql, qr = rand_state(rng, 1.2, 0.6), rand_state(rng, 1.2, 0.6)
+        shift = rng.uniform(-3.0, 3.0)
+        ql[IU] += shift
+        qr[IU] += shift
+        want, region, scale, dens = hlld_jump_form(ql, qr, p.gamma0)
+        b2 = (0.5 * (ql[IA] + qr[IA])) ** 2
+        # away from the reference's degenerate-state branch and the (S - SM) poles
+        if b2 < 1e-4 or any(abs(d) < 1e-3 * b2 or abs(d / b2 - 1) < 1e-3 for d in dens):
+            continue
+        f = oracle.riemann_mhd(p, ql.copy(), qr.copy())
+        np.testing.assert_allclose(f, want, rtol=0, atol=1e-12 * scale)
+        hits[region] += 1
+    assert np.all(hits >= 5), hits
+
+
 # ------------------------------------------------------------------ 2D EMF solver
 
 def test_emfz_uniform_state_is_v_cross_b(oracle):
