@@ -281,6 +281,71 @@ def test_emfz_field_reversal_and_rotation(oracle):
         np.testing.assert_allclose(oracle.compute_emfz(p, qr), E, rtol=1e-10, atol=1e-13)
 
 
+SHIFTS = [(0.0, 0.0), (4.0, 0.0), (-4.0, 0.0), (0.0, 4.0), (0.0, -4.0), (4.0, 4.0), (-4.0, 3.0), (1.0, -0.5)]
+
+
+def edge_states(rng, shift):
+    """Four random edge states around a corner (RT, RB, LT, LB), moved by a common velocity
+    (large shifts make one or both directions supersonic: the one-sided branches)."""
+    qe = np.array([rand_state(rng) for _ in range(4)])
+    qe[:, IU] += shift[0]
+    qe[:, IV] += shift[1]
+    return qe
+
+
+def test_emfz_mirror_symmetries(oracle):
+    # x -> -x (u and Bx change sign; the cells left and right of the corner trade places, so
+    # RT <-> LT and RB <-> LB) and y -> -y (v, By; RT <-> RB, LT <-> LB) both reverse Ez = u By -
+    # v Bx. Every branch of the solver (subsonic, supersonic along x, y or both) is exercised.
+    p = P(oracle)
+    rng = np.random.default_rng(12)
+    Mx = np.array([1, 1, -1, 1, 1, -1, 1, 1.0])
+    My = np.array([1, 1, 1, -1, 1, 1, -1, 1.0])
+    for shift in SHIFTS:
+        for _ in range(25):
+            qe = edge_states(rng, shift)
+            RT, RB, LT, LB = qe
+            E = oracle.compute_emfz(p, qe)
+            Ex = oracle.compute_emfz(p, np.array([LT, LB, RT, RB]) * Mx)
+            Ey = oracle.compute_emfz(p, np.array([RB, RT, LB, LT]) * My)
+            np.testing.assert_allclose(Ex, -E, rtol=1e-10, atol=1e-13)
+            np.testing.assert_allclose(Ey, -E, rtol=1e-10, atol=1e-13)
+
+
+def test_emfz_is_dissipative(oracle):
+    # Ez is the x-flux of By and minus the y-flux of Bx. About a uniform state, an upwind
+    # (dissipative) flux can only fall when the field behind the interface rises: dEz/dBy of
+    # the cells right of the corner < 0, of the cells left of it > 0, dEz/dBx of the cells
+    # above > 0, below < 0 — in every direction that is subsonic. Along a supersonic
+    # direction Ez takes the upwind side: with v > 0 large it falls with the lower cells' Bx
+    # (-v Bx), with u > 0 large it rises with the left cells' By (u By).
+    p = P(oracle)
+    rng = np.random.default_rng(13)
+    d = 1e-6
+    # edge-state indices (RT, RB, LT, LB) and component of each perturbation:
+    # the corner's right cells hold LT and LB, the left cells RT and RB; upper cells RB and LB
+    moves = {"By_right": ([2, 3], IB), "By_left": ([0, 1], IB), "Bx_top": ([1, 3], IA), "Bx_bottom": ([0, 2], IA)}
+    cases = {  # (u, v) of the uniform state: expected sign of each derivative (0: not checked)
+        (0.0, 0.0): {"By_right": -1, "By_left": 1, "Bx_top": 1, "Bx_bottom": -1},
+        (0.0, 6.0): {"By_right": -1, "By_left": 1, "Bx_bottom": -1},
+        (0.0, -6.0): {"By_right": -1, "By_left": 1, "Bx_top": 1},
+        (6.0, 0.0): {"Bx_top": 1, "Bx_bottom": -1, "By_left": 1},
+        (-6.0, 0.0): {"Bx_top": 1, "Bx_bottom": -1, "By_right": -1},
+    }
+    for (u, v), want in cases.items():
+        for _ in range(20):
+            q = rand_state(rng, 1.0, 0.0)
+            q[IU], q[IV] = u, v
+            base = np.tile(q, (4, 1))
+            E0 = oracle.compute_emfz(p, base)
+            for name, sign in want.items():
+                rows, comp = moves[name]
+                qe = base.copy()
+                qe[rows, comp] += d
+                dE = (oracle.compute_emfz(p, qe) - E0) / d
+                assert sign * dE > 1e-3, (u, v, name, dE)
+
+
 # ------------------------------------------------------------------ trace
 
 def test_trace_uniform_state(oracle):
