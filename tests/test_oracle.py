@@ -139,6 +139,58 @@ def test_riemann_consistency(oracle, rs, dim):
                                    rtol=2e-13, atol=2e-13)
 
 
+def hllc_jump_form(ql, qr, g):
+    """HLLC flux as published (Toro, Riemann Solvers and Numerical Methods for Fluid
+    Dynamics, 3rd ed., 2009, sec. 10.4, with the star speed of Batten et al. 1997):
+    star states U*K = rhoK (SK - uK) / (SK - S*) (1, S*, vt_K, E_K / rhoK + (S* - uK)
+    (S* + pK / (rhoK (SK - uK)))), fluxes F*K = FK + SK (U*K - UK), wave speeds
+    SL = min(uL, uR) - max(cL, cR), SR = max(uL, uR) + max(cL, cR).
+    Conservative order (rho, E, rho u, rho vt...). Returns (flux, region 0..3)."""
+    out = []
+    for q in (ql, qr):
+        r, p, u, vt = q[0], q[1], q[2], np.asarray(q[3:], float)
+        E = p / (g - 1) + 0.5 * r * (u * u + vt @ vt)
+        out.append(dict(r=r, p=p, u=u, vt=vt, E=E, c=math.sqrt(g * p / r),
+                        U=np.array([r, E, r * u, *(r * vt)]), F=physical_flux(q, g)))
+    L, R = out
+    SL = min(L["u"], R["u"]) - max(L["c"], R["c"])
+    SR = max(L["u"], R["u"]) + max(L["c"], R["c"])
+    Ss = ((R["p"] - L["p"] + L["r"] * L["u"] * (SL - L["u"]) - R["r"] * R["u"] * (SR - R["u"]))
+          / (L["r"] * (SL - L["u"]) - R["r"] * (SR - R["u"])))
+    Fs = []
+    for K, S in ((L, SL), (R, SR)):
+        f = K["r"] * (S - K["u"]) / (S - Ss)
+        Us = f * np.array([1.0, K["E"] / K["r"] + (Ss - K["u"]) * (Ss + K["p"] / (K["r"] * (S - K["u"]))), Ss, *K["vt"]])
+        Fs.append(K["F"] + S * (Us - K["U"]))
+    region = int(np.searchsorted([SL, Ss, SR], 0.0))
+    scale = max(np.abs(L["F"]).max(), np.abs(R["F"]).max(), abs(SL) * np.abs(L["U"]).max(), abs(SR) * np.abs(R["U"]).max())
+    return [L["F"], Fs[0], Fs[1], R["F"]][region], region, scale
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hllc_is_the_published_jump_form(oracle, dim):
+    # riemann_hllc (RiemannSolvers.h:445-568) evaluates a flux from the sampled star state and
+    # carries the transverse velocity by the sign of the mass flux; the published solver
+    # states each star flux through the jump conditions. On random states shifted so that
+    # all four regions of the fan are sampled the two agree to rounding.
+    solver = "Hydro_Muscl_2D" if dim == 2 else "Hydro_Muscl_3D"
+    p = P(oracle, solver, nz=16 if dim == 3 else 1, riemann="hllc")
+    rng = np.random.default_rng(9)
+    hits = np.zeros(4, int)
+    for _ in range(300):
+        ql = np.array([rng.uniform(0.2, 2), rng.uniform(0.2, 2), *rng.uniform(-1, 1, dim)])
+        qr = np.array([rng.uniform(0.2, 2), rng.uniform(0.2, 2), *rng.uniform(-1, 1, dim)])
+        shift = rng.uniform(-3, 3)
+        ql[2] += shift
+        qr[2] += shift
+        want, region, scale = hllc_jump_form(ql, qr, p.gamma0)
+        f = flux_oracle(oracle, p, ql, qr)
+        # oracle order (rho, E, rho u, rho v[, rho w]) is the conservative order used above
+        np.testing.assert_allclose(f, want, rtol=0, atol=1e-12 * scale)
+        hits[region] += 1
+    assert np.all(hits >= 10), hits
+
+
 @pytest.mark.parametrize("rs", ["hllc", "approx", "hll", "llf"])
 def test_riemann_mirror_symmetry(oracle, rs):
     p = P(oracle, riemann=rs)
