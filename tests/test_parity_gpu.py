@@ -69,6 +69,22 @@ def test_hydro2d_c1_full_fp64_parity(oracle):
     assert rel_err(g, o) <= RTOL
 
 
+@pytest.mark.parametrize("dim,rs", [(2, "hllc"), (2, "hll"), (3, "hllc"), (3, "approx")])
+def test_first_order_slope_type_0_fp64_parity(oracle, dim, rs):
+    # slope_type 0: every limited slope is zero (the reference reaches it through 0 * (q - q-));
+    # the kernels skip the limiter. A state at rest outside a blast keeps exact zeros in it.
+    if dim == 2:
+        ekp = make_params("Hydro_Muscl_2D", 72, 56, riemann=rs, slope_type="0", cfl="0.8", bc=(1, 2, 3, 3))
+    else:
+        ekp = make_params("Hydro_Muscl_3D", 40, 36, 28, riemann=rs, slope_type="0", cfl="0.5",
+                          bc=(3, 3, 1, 2, 2, 1))
+    U0 = oracle.blast(oracle.from_ek(ekp), radius=0.2, center_x=0.45, center_y=0.55, center_z=0.5,
+                      density_in=1.0, density_out=1.0, pressure_in=10.0, pressure_out=0.1)
+    g, o = run_pair(oracle, ekp, U0, 12)
+    print(f"case=slope_type 0 dim={dim} rs={rs}: max|diff|={np.max(np.abs(g - o)):.3e}", flush=True)
+    assert np.array_equal(g, o)
+
+
 @pytest.mark.parametrize("rs", ["hllc", "approx", "llf"])
 def test_hydro3d_blast_fp64_parity(oracle, rs):
     ekp = make_params("Hydro_Muscl_3D", 32, 32, 32, riemann=rs, slope_type="2", cfl="0.5",
