@@ -89,9 +89,9 @@ class _Pinned:
             raise MemoryError(f"ek_host_alloc({nbytes}) failed ({rc})")
 
     def __del__(self):
-        if self.ptr:
-            lib().ek_host_free(self.ptr)
-            self.ptr = C.c_void_p()
+        ptr, self.ptr = self.ptr, C.c_void_p()
+        if ptr and _LIB is not None:  # at interpreter exit the binding may already be gone
+            _LIB.ek_host_free(ptr)
 
 
 def pinned_empty(shape, dtype=np.float64) -> np.ndarray:
