@@ -41,6 +41,7 @@ class Job:
     admit_t: float = -1.0
     done_t: float = -1.0
     result: np.ndarray = field(default=None, repr=False)
+    error: str = None  # why the job failed (it was rejected or its context could not be made)
 
     @property
     def cells(self) -> int:
@@ -152,18 +153,24 @@ class ContextGroup:
         jid = self.next_id
         self.next_id += 1
         self.jobs[jid] = dict(ctx=None, fut=self._up.submit(self._create, p, U), t=0.0, it=0, nStepmax=nStepmax,
-                              tEnd=tEnd, need=need)
+                              tEnd=tEnd, need=need, error=None)
         self.used += need
         return jid
 
     @staticmethod
     def _ready(j) -> bool:
-        if j["ctx"] is None and j["fut"].done():
-            j["ctx"] = j["fut"].result()
+        # a context that could not be created fails its job alone (failed() reports why)
+        if j["ctx"] is None and j["error"] is None and j["fut"].done():
+            exc = j["fut"].exception()
+            if exc is None:
+                j["ctx"] = j["fut"].result()
+            else:
+                j["error"] = exc
         return j["ctx"] is not None
 
     def step(self, nsteps=1):
-        live = [j for j in self.jobs.values() if j["it"] < j["nStepmax"] and j["t"] < j["tEnd"] - 1e-14]
+        live = [j for j in self.jobs.values()
+                if j["error"] is None and j["it"] < j["nStepmax"] and j["t"] < j["tEnd"] - 1e-14]
         todo = [j for j in live if self._ready(j)]
         if not todo and live:
             # nothing uploaded yet: wait for the first context instead of spinning
@@ -180,19 +187,30 @@ class ContextGroup:
 
     def status(self, jid):
         j = self.jobs[jid]
-        return j["t"], j["it"], j["it"] >= j["nStepmax"] or j["t"] >= j["tEnd"] - 1e-14
+        self._ready(j)
+        done = j["error"] is not None or j["it"] >= j["nStepmax"] or j["t"] >= j["tEnd"] - 1e-14
+        return j["t"], j["it"], done
+
+    def failed(self, jid):
+        """The exception that failed the job, or None."""
+        j = self.jobs[jid]
+        self._ready(j)
+        return j["error"]
 
     def download(self, jid):
         j = self.jobs[jid]
         if j["ctx"] is None:
-            j["ctx"] = j["fut"].result()
+            j["ctx"] = j["fut"].result()  # raises the job's error
         return j["ctx"].download(native.EK_LAYOUT_SOA)
 
     def retire(self, jid):
         j = self.jobs.pop(jid)
-        ctx = j["ctx"] if j["ctx"] is not None else j["fut"].result()
-        ctx.close()
         self.used -= j["need"]
+        try:
+            ctx = j["ctx"] if j["ctx"] is not None else j["fut"].result()
+        except Exception:
+            return  # the context was never made
+        ctx.close()
 
     def close(self):
         for jid in list(self.jobs):
@@ -248,6 +266,11 @@ class Server:
                     jid = pool.admit(j.params, j.U0, j.steps)
                 except PoolFull:
                     break  # first come, first served: wait for memory
+                except native.EkError as e:  # rejected (e.g. invalid parameters): fail this job only
+                    pending.popleft()
+                    j.error = str(e)
+                    j.done_t = clock() - t0
+                    continue
                 j.admit_t = clock() - t0
                 active[jid] = j
                 pending.popleft()
@@ -270,7 +293,10 @@ class Server:
                 _, it, fin = pool.status(jid)
                 if fin:
                     j = active.pop(jid)
-                    if keep_results:
+                    err = pool.failed(jid) if hasattr(pool, "failed") else None
+                    if err is not None:
+                        j.error = str(err)
+                    elif keep_results:
                         j.result = pool.download(jid)
                     pool.retire(jid)
                     j.done_t = done_t
@@ -278,13 +304,20 @@ class Server:
 
 
 def summarize(jobs, launches=0):
-    tts = np.array([j.done_t - j.arrival for j in jobs])
-    first = min(j.arrival for j in jobs)
-    last = max(j.done_t for j in jobs)
-    cells = sum(j.cells * j.steps for j in jobs)
-    return {"jobs": len(jobs), "jobs_per_s": len(jobs) / (last - first), "cell_updates_per_s": cells / (last - first),
-            "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
-            "tts_max_s": float(tts.max()), "span_s": last - first, "pool_launches": launches}
+    """Throughput and time-to-solution percentiles over the jobs that completed; failed jobs
+    are counted apart."""
+    ok = [j for j in jobs if j.error is None]
+    out = {"jobs": len(ok), "failed": len(jobs) - len(ok), "pool_launches": launches}
+    if not ok:
+        return out
+    tts = np.array([j.done_t - j.arrival for j in ok])
+    first = min(j.arrival for j in ok)
+    last = max(j.done_t for j in ok)
+    cells = sum(j.cells * j.steps for j in ok)
+    out.update({"jobs_per_s": len(ok) / (last - first), "cell_updates_per_s": cells / (last - first),
+                "tts_p50_s": float(np.percentile(tts, 50)), "tts_p99_s": float(np.percentile(tts, 99)),
+                "tts_max_s": float(tts.max()), "span_s": last - first})
+    return out
 
 
 def serve_one_at_a_time(jobs, precision=native.EK_FP64, clock=time.perf_counter, sleep=time.sleep):

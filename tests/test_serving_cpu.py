@@ -1,7 +1,10 @@
-"""Host side of serving: the seeded Poisson stream and the 3D Kelvin-Helmholtz state."""
+"""Host side of serving: the seeded Poisson stream, the 3D Kelvin-Helmholtz state, and the
+serving loop's handling of jobs that fail."""
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
-from euler_kokkos_amd import serving
+from euler_kokkos_amd import native, serving
 
 
 def test_poisson_stream_is_seeded_and_in_range():
@@ -31,3 +34,86 @@ def test_kh3d_state():
     pr = (p.gamma0 - 1) * (U[1] - 0.5 * (U[2] ** 2 + U[3] ** 2) / U[0])
     assert np.allclose(pr, 2.5, rtol=1e-12)
     assert np.array_equal(U, serving.kh3d(j.params, 7))
+
+
+class _FakePool:
+    """Stands in for a job pool in the serving loop: a job's params say how it behaves
+    ("ok", "reject": refused at admission, "fail": its context fails after admission)."""
+
+    def __init__(self, async_failures):
+        self.jobs = {}
+        self.next = 0
+        self.async_failures = async_failures
+
+    def admit(self, params, U, steps):
+        if params == "reject":
+            raise native.EkError("ek_create failed (2): invalid parameters")
+        jid, self.next = self.next, self.next + 1
+        self.jobs[jid] = dict(params=params, it=0, steps=steps)
+        return jid
+
+    def step(self, n):
+        for j in self.jobs.values():
+            if j["params"] == "ok":
+                j["it"] = min(j["steps"], j["it"] + n)
+
+    def status(self, jid):
+        j = self.jobs[jid]
+        return 0.0, j["it"], j["it"] >= j["steps"] or j["params"] == "fail"
+
+    def download(self, jid):
+        return np.full(3, float(jid))
+
+    def retire(self, jid):
+        del self.jobs[jid]
+
+
+class _FakeGroup(_FakePool):
+    def failed(self, jid):
+        return RuntimeError("context creation failed") if self.jobs[jid]["params"] == "fail" else None
+
+
+def test_serving_loop_fails_bad_jobs_alone():
+    # a job refused at admission and a job whose context fails later end with an error; the
+    # others complete with their results, and the summary counts the failures apart
+    S = serving.Server.__new__(serving.Server)
+    S.pools = {"hydro_kh": _FakePool(False), "mhd_blast": _FakeGroup(True)}
+    S.max_chunk = 4
+    S.precision = native.EK_FP32
+    S._ex = ThreadPoolExecutor(max_workers=2)
+    kinds = ["hydro_kh", "mhd_blast"]
+    behaviour = ["ok", "reject", "ok", "fail", "ok", "ok"]
+    jobs = [serving.Job(i, kinds[i % 2], 8, 5, 0.001 * i, i, params=b) for i, b in enumerate(behaviour)]
+    try:
+        stats = S.serve(jobs, keep_results=True)
+    finally:
+        S._ex.shutdown()
+    assert stats["jobs"] == 4 and stats["failed"] == 2
+    for j, b in zip(jobs, behaviour):
+        assert j.done_t >= 0
+        if b == "ok":
+            assert j.error is None and j.result is not None
+        else:
+            assert j.error and j.result is None
+    assert not S.pools["hydro_kh"].jobs and not S.pools["mhd_blast"].jobs
+
+
+def test_context_group_job_that_cannot_be_created_fails_alone():
+    # ek_create refuses the 3D MHD trace's slope_type 0 before touching the device: the
+    # job is reported failed, the group keeps serving and its memory share comes back
+    from euler_kokkos_amd.params import make_params
+    G = serving.ContextGroup(1 << 40)
+    try:
+        p = make_params("MHD_Muscl_3D", 8, 8, 8, slope_type="0", riemann="hlld", gamma0="1.6666667")
+        U = np.zeros((8, 8, 8, 8))
+        jid = G.admit(p, U, 3)
+        assert G.used > 0
+        G.step(1)  # waits for the creation attempt
+        _, it, fin = G.status(jid)
+        assert fin and it == 0
+        err = G.failed(jid)
+        assert isinstance(err, native.EkError) and "slope_type" in str(err)
+        G.retire(jid)
+        assert G.used == 0 and not G.jobs
+    finally:
+        G.close()
