@@ -191,6 +191,31 @@ def test_hllc_is_the_published_jump_form(oracle, dim):
     assert np.all(hits >= 10), hits
 
 
+def test_two_shock_solver_is_exact_for_an_isolated_shock(oracle):
+    # riemann_approx (RiemannSolvers.h:59-199) iterates the two-shock approximation, whose
+    # Lagrangian wave speeds are the exact shock mass fluxes. For left/right states joined by
+    # one moving left-facing shock (Rankine-Hugoniot, upstream Mach number M in the shock
+    # frame, shock speed s) the exact Godunov state at the interface is the left state when
+    # s > 0 and the right one when s < 0, so the flux is that side's physical flux -- whichever
+    # way the fluid moves across the interface.
+    p = P(oracle, riemann="approx")
+    g = p.gamma0
+    rng = np.random.default_rng(10)
+    for _ in range(60):
+        r1, p1 = rng.uniform(0.3, 2.0), rng.uniform(0.2, 2.0)
+        M, s, v = rng.uniform(1.2, 6.0), rng.choice([-1, 1]) * rng.uniform(0.05, 1.5), rng.uniform(-1, 1)
+        c1 = math.sqrt(g * p1 / r1)
+        w1 = M * c1                                   # upstream speed relative to the shock
+        r2 = r1 * (g + 1) * M * M / ((g - 1) * M * M + 2)
+        p2 = p1 * (2 * g * M * M - (g - 1)) / (g + 1)
+        w2 = w1 * r1 / r2
+        ql = np.array([r1, p1, w1 + s, v])
+        qr = np.array([r2, p2, w2 + s, v])
+        f = flux_oracle(oracle, p, ql, qr)
+        want = physical_flux(ql if s > 0 else qr, g)
+        np.testing.assert_allclose(f, want, rtol=1e-9, atol=1e-9 * np.abs(want).max())
+
+
 @pytest.mark.parametrize("rs", ["hllc", "approx", "hll", "llf"])
 def test_riemann_mirror_symmetry(oracle, rs):
     p = P(oracle, riemann=rs)
