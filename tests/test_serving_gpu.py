@@ -57,3 +57,27 @@ def test_admission_waits_for_memory():
         assert a1 >= d0 - 1e-9  # never two jobs resident together
     for j in jobs:
         assert np.array_equal(j.result, alone(j))
+
+
+def test_bad_jobs_fail_alone_in_the_mix():
+    # one hydro job the pool refuses at admission and one MHD job whose context cannot be
+    # created (slope_type 0 is not an MHD trace option): both end with an error, every other
+    # job's result is still bitwise its result alone
+    jobs = small_stream(seed=11)
+    bad = {next(j.id for j in jobs if j.kind == "hydro_kh"), next(j.id for j in jobs if j.kind == "mhd_blast")}
+    for j in jobs:
+        if j.id in bad:
+            j.params = serving.job_params(j.kind, j.n, native.EK_FP32)
+            j.params.slope_type = 0.0
+    S = serving.Server(256 << 20)
+    try:
+        stats = S.serve(jobs, keep_results=True)
+    finally:
+        S.close()
+    print(f"case=serving mix with 2 bad jobs: {stats}", flush=True)
+    assert stats["failed"] == 2 and stats["jobs"] == len(jobs) - 2
+    for j in jobs:
+        if j.id in bad:
+            assert j.error and j.result is None
+        else:
+            assert j.error is None and np.array_equal(j.result, alone(j)), j.id
