@@ -138,6 +138,7 @@ def mhd3d_c5(n, steps, with_cpu, cpu_seconds):
                      "mass_drift_rel": float(abs(b1[0] - b0[0]) / abs(b0[0])),
                      "energy_drift_rel": float(abs(b1[1] - b0[1]) / abs(b0[1]))}
     res["fp32"]["l1_rho_rel"] = float(np.abs(out["fp32"][0] - out["fp64"][0]).mean() / np.abs(out["fp64"][0]).mean())
+    res["fp32"]["l1_E_rel"] = float(np.abs(out["fp32"][1] - out["fp64"][1]).mean() / np.abs(out["fp64"][1]).mean())
     if with_cpu:
         from oracle import pyoracle
         ekp = job_params("mhd_blast", n, native.EK_FP64)
@@ -249,7 +250,7 @@ def batch_c4(n_runs, steps, seed):
             wall = time.perf_counter() - t0
             ms, launches = P.last_step_time()
             assert all(P.status(j)[1] == steps for j in ids)
-            finals[name] = [P.download(j)[0] for j in ids]
+            finals[name] = [P.download(j)[:2] for j in ids]  # density and total energy
         finally:
             P.close()
         res[name] = {"runs_per_s": n_runs / wall, "cell_updates_per_s": cells * steps / wall,
@@ -279,9 +280,11 @@ def batch_c4(n_runs, steps, seed):
                                  "tts_p99_s": float(np.percentile(done, 99)),
                                  "run_time_p50_s": float(np.percentile(per, 50)),
                                  "run_time_p99_s": float(np.percentile(per, 99))}
-    l1 = [float(np.abs(a - b).mean() / np.abs(b).mean()) for a, b in zip(finals["fp32_batched"], finals["fp64_batched"])]
-    res["fp32_l1_rho_rel_max"] = max(l1)
-    res["fp32_l1_rho_rel_mean"] = float(np.mean(l1))
+    for v, tag in ((0, "rho"), (1, "E")):
+        l1 = [float(np.abs(a[v] - b[v]).mean() / np.abs(b[v]).mean())
+              for a, b in zip(finals["fp32_batched"], finals["fp64_batched"])]
+        res[f"fp32_l1_{tag}_rel_max"] = max(l1)
+        res[f"fp32_l1_{tag}_rel_mean"] = float(np.mean(l1))
     res["fp32_batched_vs_fp64_one_at_a_time"] = (res["fp32_batched"]["cell_updates_per_s"]
                                                  / res["fp64_one_at_a_time"]["cell_updates_per_s"])
     return res
@@ -320,6 +323,26 @@ def serving_c5(n_jobs, rate, seed, steps):
             j.done_t = -1.0
         res[name] = {"fp32_continuous_batching": fp32, "fp64_one_at_a_time": fp64,
                      "fp32_batched_vs_fp64_one_at_a_time": fp32["cell_updates_per_s"] / fp64["cell_updates_per_s"]}
+    # accuracy of the served FP32 results against FP64, outside the timed runs: a served job's
+    # result is bitwise its result alone (tests/test_serving_gpu.py), so each job is rerun alone
+    # in FP32 and in FP64 and the relative L1 of density and total energy compared
+    l1 = {"rho": [], "E": []}
+    for j in jobs:
+        fin = {}
+        for prec in (native.EK_FP32, native.EK_FP64):
+            ctx = native.Context(serving.job_params(j.kind, j.n, prec))
+            try:
+                ctx.upload(j.U0, native.EK_LAYOUT_SOA)
+                ctx.run(j.steps, j.steps, 1e30)
+                fin[prec] = ctx.download(native.EK_LAYOUT_SOA)
+            finally:
+                ctx.close()
+        for v, tag in ((0, "rho"), (1, "E")):
+            a, b = fin[native.EK_FP32][v], fin[native.EK_FP64][v]
+            l1[tag].append(float(np.abs(a - b).mean() / np.abs(b).mean()))
+    for tag, vals in l1.items():
+        res[f"fp32_l1_{tag}_rel_max"] = max(vals)
+        res[f"fp32_l1_{tag}_rel_mean"] = float(np.mean(vals))
     return res
 
 
