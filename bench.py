@@ -56,6 +56,7 @@ def gpu_run(ekp, U0, warmup, steps):
     ctx = native.Context(ekp)
     try:
         ctx.upload(U0, native.EK_LAYOUT_SOA)
+        b0 = ctx.budgets()
         t, it, _ = ctx.run(warmup, warmup + steps, 1e30)
         ctx.run(0, warmup + steps, 1e30, t, it)  # capture the step graph outside the timed region
         device_sync()
@@ -68,7 +69,7 @@ def gpu_run(ekp, U0, warmup, steps):
         b = ctx.budgets()
     finally:
         ctx.close()
-    return dict(wall=wall, ev_ms=ev_ms, launches=launches, U=U, t=t, it=it, dt=dt, budgets=b)
+    return dict(wall=wall, ev_ms=ev_ms, launches=launches, U=U, t=t, it=it, dt=dt, budgets=b, budgets0=b0)
 
 
 def cpu_baseline(n, seconds):
@@ -429,6 +430,13 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                      "kernel": "hydro3d_step<double,16,8,256,HLLC,MC>",
                      "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes},
+        # totals over the periodic grid before the warm-up and after the timed steps (FP64
+        # compensated sums); the blast starts at rest, so momentum drift is given per unit mass
+        "conservation": {"steps": a.warmup + a.steps,
+                         "mass_drift_rel": abs(r64["budgets"][0] - r64["budgets0"][0]) / abs(r64["budgets0"][0]),
+                         "energy_drift_rel": abs(r64["budgets"][1] - r64["budgets0"][1]) / abs(r64["budgets0"][1]),
+                         "momentum_drift_per_mass": max(abs(r64["budgets"][v] - r64["budgets0"][v]) for v in (2, 3, 4))
+                         / abs(r64["budgets0"][0])},
     }
     if not a.no_fp32:
         ekp32 = make_params("Hydro_Muscl_3D", n, n, n, precision=native.EK_FP32, **kw)
