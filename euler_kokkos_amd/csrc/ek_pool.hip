@@ -89,10 +89,15 @@ static int kind_dim(const ek_pool* p)
   return p->kind.dim;
 }
 
+// z-chunk length of a 3D job: the tile choice's depth, with the job's planes then spread
+// evenly over that many chunks (a chunk's block lasts as long as its chunk is deep)
 static int cz_of(const ek_pool* p, const ek_params& j)
 {
   const int c = kCz[p->variant & 3];
-  return c < j.nz ? c : j.nz;
+  if (c >= j.nz)
+    return j.nz;
+  const int nch = (j.nz + c - 1) / c;
+  return (j.nz + nch - 1) / nch;
 }
 
 // PoolJob record of job `id` for the current tile choice
