@@ -96,10 +96,11 @@ def test_hydro3d_blast_fp64_parity(oracle, rs):
     assert rel_err(g, o) <= RTOL
 
 
-@pytest.mark.parametrize("shape,cz", [((256, 128, 60), 8), ((192, 192, 100), 12)])
+@pytest.mark.parametrize("shape,cz", [((256, 128, 60), 8), ((192, 192, 100), 12), ((256, 256, 90), 23)])
 def test_hydro3d_z_chunk_lengths_fp64_parity(oracle, shape, cz):
     # grids whose z-chunk length (chosen from the grid size) is 8 or 12 with a partial last
-    # chunk: the chunking changes where planes are traced, never a result
+    # chunk, or 23 (24 spread evenly over 4 chunks): the chunking changes where planes are
+    # traced, never a result
     ekp = make_params("Hydro_Muscl_3D", *shape, riemann="hllc", slope_type="2", cfl="0.5", bc=(3, 3, 1, 1, 2, 2))
     U0 = oracle.blast(oracle.from_ek(ekp), radius=0.2, center_x=0.45, center_y=0.55, center_z=0.5,
                       density_in=1.0, density_out=1.0, pressure_in=10.0, pressure_out=0.1)
