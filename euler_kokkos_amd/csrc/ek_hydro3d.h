@@ -278,7 +278,12 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
       if (p + 2 <= k1 + 1)
         fetch(p + 2, upre);
     }
-    // B: slopes and face states of plane p
+    // B: slopes and face states of plane p. The two tile-cell waves trace six faces per
+    // cell, the halo waves one or none and then wait at the barrier: the tile-cell waves
+    // take issue priority until it (FP64 256^3: 1.190 -> 1.175 ms per step)
+    const bool heavy = __builtin_amdgcn_readfirstlane(wave) < 2;
+    if (heavy)
+      __builtin_amdgcn_s_setprio(2);
     if (want) {
       unsigned w = want;
       if (warm || last)
@@ -347,6 +352,8 @@ __device__ __forceinline__ double hydro3d_column(const T* __restrict__ Uin, T* _
           trace_face<T, NV>(gas, qc, dq[2], s, dtdx[2], true, qzm);
       }
     }
+    if (heavy)
+      __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     // W: plane p+1 becomes the LDS plane (every read of plane p is done)
     if (haspos && !last) {
