@@ -26,6 +26,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads stay on the cores they start on (read by the OpenMP
+# runtime when the oracle library loads); the thread count is the environment's
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 METRIC = ("cell-updates/s at 1 GPU, % HBM peak; batched runs/s + p50/p99 at FP32 vs FP64")
 HBM_PEAK_GBS = 8000.0
@@ -91,10 +94,25 @@ def cpu_baseline(n, seconds):
     cores = pyoracle.lib().eko_num_threads()
     cups = steps * n ** 3 / el
     return {"value": cups, "unit": "cell-updates/s", "cores": int(cores), "kind": "port",
-            "sample": f"oracle (C restatement of the reference impl-0 OpenMP step, -O2, no FMA) "
-                      f"on the same {n}^3 blast, {steps} steps after 1 untimed step, "
+            "sample": f"oracle (C restatement of the reference impl-0 OpenMP step, gcc -O3 "
+                      f"-ffp-contract=off) on the same {n}^3 blast, {steps} steps after 1 untimed step, "
                       f"{cores} OpenMP threads",
-            "ref_style_mcell_per_s": steps * (n + 4) ** 3 / el * 1e-6}
+            "ref_style_mcell_per_s": steps * (n + 4) ** 3 / el * 1e-6, **host_info()}
+
+
+def host_info():
+    """CPU model and OpenMP placement of the host the CPU baseline runs on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+            "host_cpus_available": len(os.sched_getaffinity(0))}
 
 
 def mhd3d_c5(n, steps, with_cpu, cpu_seconds):
@@ -157,7 +175,7 @@ def mhd3d_c5(n, steps, with_cpu, cpu_seconds):
         cores = pyoracle.lib().eko_num_threads()
         res["cpu_baseline"] = {"value": k * n ** 3 / el, "unit": "cell-updates/s", "cores": int(cores),
                                "kind": "port", "sample": f"oracle 3D MHD step on the same {n}^3 blast, "
-                               f"{k} steps after 1 untimed step, {cores} OpenMP threads"}
+                               f"{k} steps after 1 untimed step, {cores} OpenMP threads", **host_info()}
     return res
 
 
@@ -221,7 +239,7 @@ def mhd_c3(steps, with_cpu, cpu_seconds):
         cores = pyoracle.lib().eko_num_threads()
         res["cpu_baseline"] = {"value": k * n * n / el, "unit": "cell-updates/s", "cores": int(cores),
                                "kind": "port", "sample": f"oracle MHD step on the same {n}^2 Orszag-Tang, "
-                               f"{k} steps after 1 untimed step, {cores} OpenMP threads"}
+                               f"{k} steps after 1 untimed step, {cores} OpenMP threads", **host_info()}
     return res
 
 
@@ -256,7 +274,10 @@ def batch_c4(n_runs, steps, seed):
             P.close()
         res[name] = {"runs_per_s": n_runs / wall, "cell_updates_per_s": cells * steps / wall,
                      "wall_s": wall, "ms_per_pool_step": ms / max(launches, 1),
+                     # every run has the same step count, so all finish at the last launch:
+                     # time-to-solution is the batch's wall time for every run
                      "tts_p50_s": wall, "tts_p99_s": wall,
+                     "tts_note": "all runs advance together and finish at the same launch",
                      "tile_choice": choice, "autotune_ms_per_step": tune_ms}
     # plain FP64, one job at a time (queue order = generation order)
     done, per = [], []
@@ -347,6 +368,15 @@ def serving_c5(n_jobs, rate, seed, steps):
     return res
 
 
+def peak_source(dev):
+    """Where the HBM peak comes from: MI355X's published 8 TB/s, beside what the runtime
+    reports (memory clock x bus width; HBM3E moves 4 bits per pin per reported clock)."""
+    derived = dev["mem_clock_khz"] * 1e3 * 4 * dev["bus_width_bits"] / 8 / 1e9
+    return (f"published MI355X HBM3E peak {HBM_PEAK_GBS:.0f} GB/s; the runtime reports a "
+            f"{dev['mem_clock_khz'] / 1e3:.0f} MHz memory clock on a {dev['bus_width_bits']}-bit bus, "
+            f"{derived:.0f} GB/s at 4 transfers per reported clock")
+
+
 def copy_calibration(gib=1.0):
     """A device-to-device copy of known size, for scaling rocprofv3 FETCH/WRITE counters."""
     hip = C.CDLL("libamdhip64.so")
@@ -398,6 +428,7 @@ def main() -> int:
 
     from euler_kokkos_amd import native, problems
     from euler_kokkos_amd.params import make_params
+    dev = native.device_info()
     n = a.n
     kw = dict(gamma0="1.4", cfl="0.5", slope_type="2", riemann="hllc", bc=(3,) * 6)
     ekp64 = make_params("Hydro_Muscl_3D", n, n, n, **kw)
@@ -429,7 +460,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(),
                      "kernel": "hydro3d_step<double,16,8,256,HLLC,MC>",
-                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes},
+                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": alg_bytes,
+                     "peak_source": peak_source(dev)},
+        "device": dev,
         # totals over the periodic grid before the warm-up and after the timed steps (FP64
         # compensated sums); the blast starts at rest, so momentum drift is given per unit mass
         "conservation": {"steps": a.warmup + a.steps,

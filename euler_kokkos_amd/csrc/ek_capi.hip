@@ -90,7 +90,8 @@ struct ek_ctx {
   int last_run_launches = 0;
 };
 
-static std::string g_create_err;
+// message of the last failed ek_create on this thread (contexts may be created concurrently)
+static thread_local std::string g_create_err;
 
 static int fail(ek_ctx* c, int code, const std::string& msg)
 {
@@ -598,4 +599,27 @@ extern "C" int ek_host_free(void* ptr)
   if (!ptr)
     return EK_OK;
   return hipHostFree(ptr) == hipSuccess ? EK_OK : EK_EHIP;
+}
+
+extern "C" int ek_device_info(int* cus, int* mem_clock_khz, int* bus_width_bits, int* sclk_khz, char* name,
+                              int name_len)
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess)
+    return EK_EHIP;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return EK_EHIP;
+  if (cus)
+    *cus = prop.multiProcessorCount;
+  if (mem_clock_khz)
+    *mem_clock_khz = prop.memoryClockRate;
+  if (bus_width_bits)
+    *bus_width_bits = prop.memoryBusWidth;
+  if (sclk_khz)
+    *sclk_khz = prop.clockRate;
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  return EK_OK;
 }

@@ -66,6 +66,7 @@ struct ek_pool {
   double* stage = nullptr;
   long long stage_elems = 0;
   double* partial = nullptr;
+  int test_fail_admit = 0; // ek_pool_test_fail_admit: 0 off, else the admission step that fails
 };
 
 static int pfail(ek_pool* p, int code, const std::string& m)
@@ -82,7 +83,8 @@ static int pfail(ek_pool* p, int code, const std::string& m)
       return pfail((p), EK_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-static std::string g_pool_create_err;
+// message of the last failed ek_pool_create on this thread
+static thread_local std::string g_pool_create_err;
 
 static int kind_dim(const ek_pool* p)
 {
@@ -158,11 +160,13 @@ static int rebuild_work(ek_pool* p)
           w.push_back(ek::PoolWork{ id, tx, ty, tz * 2 + ((tx | ty | tz) == 0 ? 1 : 0) });
   }
   if ((int)w.size() > p->work_cap) {
-    hipFree(p->dwork);
-    p->dwork = nullptr;
-    p->work_cap = 0;
+    // the new list buffer first: on failure the previous list (and nwork) stay valid
     const int cap = (int)std::max<size_t>(w.size() * 2, 1024);
-    PHIP(p, hipMalloc(&p->dwork, (size_t)cap * sizeof(ek::PoolWork)));
+    ek::PoolWork* nw = nullptr;
+    PHIP(p, hipMalloc(&nw, (size_t)cap * sizeof(ek::PoolWork)));
+    PHIP(p, hipStreamSynchronize(p->stream)); // no launch still reads the old list
+    hipFree(p->dwork);
+    p->dwork = nw;
     p->work_cap = cap;
   }
   if (!w.empty())
@@ -228,7 +232,9 @@ static int refresh_status(ek_pool* p, bool& changed)
   const int slot = (int)(p->nstep % 3);
   for (int id = 0; id < (int)p->jobs.size(); id++) {
     HostJob& j = p->jobs[id];
-    if (!j.used)
+    // a finished job has left the work list: its slots are no longer forwarded, so its
+    // t / it / finished stay as the refresh that saw it finish left them
+    if (!j.used || j.finished)
       continue;
     const ek::TimeSlot& s = h[(size_t)id * 3 + slot];
     j.t = s.t;
@@ -402,58 +408,29 @@ extern "C" int ek_pool_active(const ek_pool* p)
   return n;
 }
 
-extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host, int layout, int nStepmax,
-                             double tEnd, int* job_id)
+// Uploads the admitted job's initial state into arena buffer `par` at its offset, resets its
+// time slots and reduces its first invDt. Touches nothing outside the job's own arena
+// range, slots and the staging buffer.
+static int admit_upload(ek_pool* p, const ek_params* jp, const double* host, int layout, int id, long long off,
+                        long long nelem)
 {
-  if (!p || !jp || !host || !job_id)
-    return pfail(p, EK_EINVAL, "null argument");
-  std::string why;
-  if (ek::validate_params(jp, why))
-    return pfail(p, EK_EINVAL, why);
-  if (jp->dim != p->kind.dim || jp->mhd != p->kind.mhd || jp->riemann != p->kind.riemann ||
-      jp->slope_type != p->kind.slope_type || jp->precision != p->kind.precision)
-    return pfail(p, EK_EINVAL, "job does not belong to the pool's kind (dim, mhd, riemann, slope_type, precision)");
-  int id = -1;
-  for (int i = 0; i < p->max_jobs; i++)
-    if (!p->jobs[i].used) {
-      id = i;
-      break;
-    }
-  if (id < 0)
-    return pfail(p, EK_EFULL, "no free job slot");
-  const long long ncell = job_cells(*jp), nelem = ncell * jp->nbvar, need = round_up(nelem);
-  const long long off = arena_alloc(p, need);
-  if (off < 0)
-    return pfail(p, EK_EFULL, "arena full");
-  HostJob& h = p->jobs[id];
-  h = HostJob{};
-  h.used = true;
-  h.p = *jp;
-  h.off = off;
-  h.nelem = nelem;
-  h.alloc = need;
-  h.ncell = ncell;
-  h.adm_step = p->nstep;
-  h.nStepmax = nStepmax;
-  h.tEnd = tEnd;
   const int par = (int)(p->nstep % 2);
   char* dst = (char*)p->A[par] + (size_t)off * p->elem;
-  int rc = EK_OK;
+  if (p->test_fail_admit == 1)
+    return pfail(p, EK_EHIP, "injected admission failure (upload)");
   if (layout == EK_LAYOUT_SOA) {
     if (p->fp64) {
       PHIP(p, hipMemcpyAsync(dst, host, (size_t)nelem * 8, hipMemcpyHostToDevice, p->stream));
-      PHIP(p, hipStreamSynchronize(p->stream));
     } else {
       // FP64 values through the device staging buffer, rounded to FP32 on the device
-      if ((rc = ensure_stage(p, std::min<long long>(nelem, 1LL << 22))))
+      if (int rc = ensure_stage(p, std::min<long long>(nelem, 1LL << 22)))
         return rc;
       PHIP(p, ek::upload_f64_as_f32(host, (float*)dst, nelem, p->stage, p->stage_elems, p->stream));
-      PHIP(p, hipStreamSynchronize(p->stream));
     }
   } else if (layout == EK_LAYOUT_REFERENCE) {
     const int is = jp->nx + 2 * jp->gw, js = jp->ny + 2 * jp->gw, ks = jp->nz + 2 * jp->gw;
     const long long nr = (long long)is * js * (jp->dim == 3 ? ks : 1) * jp->nbvar;
-    if ((rc = ensure_stage(p, nr)))
+    if (int rc = ensure_stage(p, nr))
       return rc;
     PHIP(p, hipMemcpyAsync(p->stage, host, (size_t)nr * 8, hipMemcpyHostToDevice, p->stream));
     if (p->fp64) {
@@ -467,12 +444,10 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
       ek::fill_geom<float>(*jp, g, gs);
       PHIP(p, ek::fp32::ref_to_soa(jp->dim, p->stage, (float*)dst, g, jp->gw, jp->nbvar, is, js, ks, p->stream));
     }
-    PHIP(p, hipStreamSynchronize(p->stream));
   } else {
-    h.used = false;
-    arena_free(p, off, need);
     return pfail(p, EK_EINVAL, "unknown layout");
   }
+  PHIP(p, hipStreamSynchronize(p->stream));
   // time slots: t = 0, it = 0 in the slot the next pool step reads, then its invDt
   ek::TimeSlot init[3];
   std::memset(init, 0, sizeof(init));
@@ -489,11 +464,75 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
     ek::fill_geom<float>(*jp, g, gs);
     PHIP(p, ek::fp32::inv_dt(jp->dim, (const float*)dst, g, gs, bits, nullptr, p->stream));
   }
-  if ((rc = upload_record(p, id)))
+  PHIP(p, hipStreamSynchronize(p->stream));
+  if (p->test_fail_admit == 2)
+    return pfail(p, EK_EHIP, "injected admission failure (after the upload)");
+  return EK_OK;
+}
+
+extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host, int layout, int nStepmax,
+                             double tEnd, int* job_id)
+{
+  if (!p || !jp || !host || !job_id)
+    return pfail(p, EK_EINVAL, "null argument");
+  std::string why;
+  if (ek::validate_params(jp, why))
+    return pfail(p, EK_EINVAL, why);
+  if (jp->dim != p->kind.dim || jp->mhd != p->kind.mhd || jp->riemann != p->kind.riemann ||
+      jp->slope_type != p->kind.slope_type || jp->precision != p->kind.precision)
+    return pfail(p, EK_EINVAL, "job does not belong to the pool's kind (dim, mhd, riemann, slope_type, precision)");
+  if (layout != EK_LAYOUT_SOA && layout != EK_LAYOUT_REFERENCE)
+    return pfail(p, EK_EINVAL, "unknown layout");
+  int id = -1;
+  for (int i = 0; i < p->max_jobs; i++)
+    if (!p->jobs[i].used) {
+      id = i;
+      break;
+    }
+  if (id < 0)
+    return pfail(p, EK_EFULL, "no free job slot");
+  const long long ncell = job_cells(*jp), nelem = ncell * jp->nbvar, need = round_up(nelem);
+  const long long off = arena_alloc(p, need);
+  if (off < 0)
+    return pfail(p, EK_EFULL, "arena full");
+  // The slot is filled but stays unused (out of the work list, status and downloads) until
+  // every step has succeeded; any failure gives the arena range and the slot back, so the
+  // pool is as it was before the call.
+  HostJob& h = p->jobs[id];
+  h = HostJob{};
+  h.p = *jp;
+  h.off = off;
+  h.nelem = nelem;
+  h.alloc = need;
+  h.ncell = ncell;
+  h.adm_step = p->nstep;
+  h.nStepmax = nStepmax;
+  h.tEnd = tEnd;
+  auto rollback = [&](int rc) {
+    p->jobs[id] = HostJob{};
+    arena_free(p, off, need);
     return rc;
-  if ((rc = rebuild_work(p)))
-    return rc;
+  };
+  if (int rc = admit_upload(p, jp, host, layout, id, off, nelem))
+    return rollback(rc);
+  if (int rc = upload_record(p, id))
+    return rollback(rc);
+  h.used = true;
+  if (p->test_fail_admit == 3) {
+    pfail(p, EK_EHIP, "injected admission failure (work list)");
+    return rollback(EK_EHIP);
+  }
+  if (int rc = rebuild_work(p))
+    return rollback(rc); // rebuild_work leaves the previous work list in place on failure
   *job_id = id;
+  return EK_OK;
+}
+
+extern "C" int ek_pool_test_fail_admit(ek_pool* p, int stage)
+{
+  if (!p || stage < 0 || stage > 3)
+    return EK_EINVAL;
+  p->test_fail_admit = stage;
   return EK_OK;
 }
 

@@ -21,7 +21,7 @@ RIEMANN = {"approx": 0, "llf": 1, "hll": 2, "hllc": 3, "hlld": 4}
 # Exported symbols declared by include/ek_hip.h (checked by the CPU test-suite).
 EXPORTS = ("ek_create", "ek_destroy", "ek_last_error", "ek_upload", "ek_download", "ek_inv_dt",
            "ek_step", "ek_run", "ek_budgets", "ek_sync", "ek_device_bytes", "ek_last_run_time",
-           "ek_host_alloc", "ek_host_free")
+           "ek_host_alloc", "ek_host_free", "ek_device_info")
 
 
 class EkParams(C.Structure):
@@ -72,11 +72,24 @@ def lib() -> C.CDLL:
     L.ek_device_bytes.restype = C.c_longlong
     L.ek_host_alloc.argtypes = [C.c_size_t, P(C.c_void_p)]
     L.ek_host_free.argtypes = [C.c_void_p]
+    L.ek_device_info.argtypes = [P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_char_p, C.c_int]
     for n in ("ek_upload", "ek_download", "ek_inv_dt", "ek_step", "ek_run", "ek_budgets", "ek_sync",
-              "ek_host_alloc", "ek_host_free"):
+              "ek_host_alloc", "ek_host_free", "ek_device_info"):
         getattr(L, n).restype = C.c_int
     _LIB = L
     return L
+
+
+def device_info() -> dict:
+    """Compute units, memory clock and bus width, engine clock and name of the current device
+    (ek_device_info)."""
+    cus, mclk, bw, sclk = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    name = C.create_string_buffer(256)
+    rc = lib().ek_device_info(C.byref(cus), C.byref(mclk), C.byref(bw), C.byref(sclk), name, 256)
+    if rc != 0:
+        raise EkError(f"ek_device_info failed ({rc})")
+    return {"cus": cus.value, "mem_clock_khz": mclk.value, "bus_width_bits": bw.value,
+            "sclk_khz": sclk.value, "name": name.value.decode(errors="replace")}
 
 
 class _Pinned:

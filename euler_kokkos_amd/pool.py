@@ -25,7 +25,7 @@ EK_EFULL = 4
 POOL_EXPORTS = ("ek_pool_create", "ek_pool_destroy", "ek_pool_last_error", "ek_pool_job_bytes",
                 "ek_pool_free_bytes", "ek_pool_active", "ek_pool_admit", "ek_pool_step",
                 "ek_pool_status", "ek_pool_download", "ek_pool_budgets", "ek_pool_retire",
-                "ek_pool_autotune", "ek_pool_set_tile", "ek_pool_last_step_time")
+                "ek_pool_autotune", "ek_pool_set_tile", "ek_pool_last_step_time", "ek_pool_test_fail_admit")
 
 _BOUND = False
 
@@ -55,6 +55,7 @@ def _lib() -> C.CDLL:
         L.ek_pool_autotune.argtypes = [V, C.c_int, P(C.c_int), D, C.c_int]
         L.ek_pool_set_tile.argtypes = [V, C.c_int]
         L.ek_pool_last_step_time.argtypes = [V, D, P(C.c_int)]
+        L.ek_pool_test_fail_admit.argtypes = [V, C.c_int]
         _BOUND = True
     return L
 
@@ -151,6 +152,11 @@ class Pool:
         ms, n = C.c_double(), C.c_int()
         self._check(self._L.ek_pool_last_step_time(self.h, C.byref(ms), C.byref(n)), "ek_pool_last_step_time")
         return ms.value, n.value
+
+    def test_fail_admit(self, stage: int) -> None:
+        """Make every following admission fail at `stage` (1 before the upload, 2 after it, 3
+        after the job's device record is written; 0 turns it off) -- for rollback tests."""
+        self._check(self._L.ek_pool_test_fail_admit(self.h, int(stage)), "ek_pool_test_fail_admit")
 
 
 # ------------------------------------------------------------------ seeded generator

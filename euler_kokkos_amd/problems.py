@@ -132,3 +132,16 @@ def soa_from_reference(p, R: np.ndarray) -> np.ndarray:
     if p.dim == 2:
         return np.ascontiguousarray(R[g:-g, g:-g, :].transpose(2, 1, 0))
     return np.ascontiguousarray(R[g:-g, g:-g, g:-g, :].transpose(3, 2, 1, 0))
+
+
+def reference_from_soa(p, S: np.ndarray) -> np.ndarray:
+    """Device SoA U[var][k][j][i] as a reference-layout array U(i,j[,k],var) with zero ghost
+    cells (a solver fills them before it reads them)."""
+    g = p.gw
+    if p.dim == 2:
+        R = np.zeros((p.nx + 2 * g, p.ny + 2 * g, p.nbvar))
+        R[g:-g, g:-g, :] = S.transpose(2, 1, 0)
+    else:
+        R = np.zeros((p.nx + 2 * g, p.ny + 2 * g, p.nz + 2 * g, p.nbvar))
+        R[g:-g, g:-g, g:-g, :] = S.transpose(3, 2, 1, 0)
+    return R

@@ -178,7 +178,12 @@ class ContextGroup:
             todo = [j for j in live if self._ready(j)]
 
         def one(j):
-            j["t"], j["it"], _ = j["ctx"].run(nsteps, j["nStepmax"], j["tEnd"], j["t"], j["it"])
+            # a job whose step fails (a HIP error on its context) fails alone: status() then
+            # reports it done and failed() returns the error
+            try:
+                j["t"], j["it"], _ = j["ctx"].run(nsteps, j["nStepmax"], j["tEnd"], j["t"], j["it"])
+            except Exception as e:  # noqa: BLE001 -- any failure of this job's context
+                j["error"] = e
 
         if len(todo) == 1:
             one(todo[0])

@@ -128,6 +128,12 @@ long long ek_device_bytes(const ek_ctx* ctx);
 int ek_host_alloc(size_t bytes, void** ptr);
 int ek_host_free(void* ptr);
 
+/* Properties of the current HIP device that measurements report: compute units, memory
+ * clock (kHz) and bus width (bits) as the runtime gives them, peak engine clock (kHz), and
+ * the device name with its architecture (name_len bytes, NUL-terminated). (No reference
+ * counterpart: measurement support.) */
+int ek_device_info(int* cus, int* mem_clock_khz, int* bus_width_bits, int* sclk_khz, char* name, int name_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,12 @@ int ek_pool_set_tile(ek_pool* pool, int choice);
 /* Device time of the step launches of the last ek_pool_step (ms) and their count. */
 int ek_pool_last_step_time(const ek_pool* pool, double* ms, int* launches);
 
+/* Failure injection for tests of admission's rollback: with stage 1, 2 or 3, every following
+ * ek_pool_admit fails (EK_EHIP) before its upload, after its upload and time-slot reset, or
+ * after its device record is written; stage 0 turns it off. A failed admission leaves the
+ * pool as it was (arena, job slots, work list). */
+int ek_pool_test_fail_admit(ek_pool* pool, int stage);
+
 #ifdef __cplusplus
 }
 #endif

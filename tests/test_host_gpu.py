@@ -7,7 +7,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from euler_kokkos_amd import native
+from euler_kokkos_amd import native, problems
 from euler_kokkos_amd.params import make_params
 
 pytestmark = pytest.mark.gpu
@@ -165,12 +165,9 @@ bz_out=0.2
 """
 
 
-def test_ini_mhd3d_blast_matches_oracle(tmp_path, oracle):
-    ini = tmp_path / "m3.ini"
-    ini.write_text(INI_MHD3D)
-    ekp = make_params("MHD_Muscl_3D", 14, 12, 16, gamma0="1.666", cfl="0.8", slope_type="2",
-                      riemann="hlld", bc=(3, 3, 1, 2, 3, 3))
-    op = oracle.from_ek(ekp)
+def mhd3d_blast_oracle_ic(oracle, op):
+    """The state InitBlastFunctor3D_MHD makes from INI_MHD3D's [blast] section (BlastParams
+    reads the bz keys into by and leaves bz at zero), every real through strtof."""
     f = lambda s: float(np.float32(s))  # ConfigMap::getFloat
     U = np.zeros(oracle.ref_shape(op))
     g = 3
@@ -184,11 +181,164 @@ def test_ini_mhd3d_blast_matches_oracle(tmp_path, oracle):
     U[..., 0] = np.where(inside, 1.0, f("1.2"))
     U[..., 5], U[..., 6] = a, b
     U[..., 1] = np.where(inside, 10.0, f("0.1")) / (op.gamma0 - 1.0) + 0.5 * (a * a + b * b + 0.0)
+    return U
+
+
+MHD3D_EKP = dict(gamma0="1.666", cfl="0.8", slope_type="2", riemann="hlld", bc=(3, 3, 1, 2, 3, 3))
+
+
+def test_ini_mhd3d_blast_matches_oracle(tmp_path, oracle):
+    ini = tmp_path / "m3.ini"
+    ini.write_text(INI_MHD3D)
+    ekp = make_params("MHD_Muscl_3D", 14, 12, 16, **MHD3D_EKP)
+    op = oracle.from_ek(ekp)
+    U = mhd3d_blast_oracle_ic(oracle, op)
     t_o, it_o, _ = oracle.mhd3d_run(op, U, 6, 1e30)
     G, t, it = ini_run(ini, 8 * 14 * 12 * 16)
     assert it == it_o == 6 and t == t_o
     want = U[3:-3, 3:-3, 3:-3, :].transpose(3, 2, 1, 0).reshape(-1)
     np.testing.assert_array_equal(G, want)
+
+
+# ---------------------------------------------------------------- VTI output content
+INI_3D = """[run]
+solver_name=Hydro_Muscl_3D
+tEnd=1e30
+nStepmax=4
+nOutput=0
+[mesh]
+nx=20
+ny=16
+nz=12
+boundary_type_xmin=1
+boundary_type_xmax=2
+boundary_type_ymin=3
+boundary_type_ymax=3
+boundary_type_zmin=2
+boundary_type_zmax=1
+[hydro]
+gamma0=1.4
+cfl=0.5
+slope_type=2
+problem=blast
+riemann=hllc
+[blast]
+radius=0.3
+center_x=0.4
+center_y=0.5
+center_z=0.55
+density_in=1.0
+density_out=1.0
+pressure_in=10.0
+pressure_out=0.1
+"""
+
+NAMES = ["rho", "energy", "rho_vx", "rho_vy", "rho_vz", "bx", "by", "bz"]
+
+
+def parse_vti(path):
+    """(extent, [(name, values)]) of a VTI file the reference's save_VTK_2D/3D layout:
+    ascii DataArrays, or raw appended data (a UInt64 byte count before each array) whose
+    offsets the DataArray elements give."""
+    import re
+    raw = open(path, "rb").read()
+    ascii_ = b'format="ascii"' in raw
+    cut = len(raw) if ascii_ else raw.index(b"<AppendedData")
+    head = raw[:cut].decode()
+    extent = [int(v) for v in re.search(r'WholeExtent="([^"]*)"', head).group(1).split()]
+    out = []
+    if ascii_:
+        for m in re.finditer(r'<DataArray type="Float64" Name="([^"]+)" format="ascii" >\n(.*?)\n    </DataArray>',
+                             head, re.S):
+            out.append((m.group(1), m.group(2).split()))
+        return extent, out
+    blob = raw[raw.index(b"_", cut) + 1:]
+    for m in re.finditer(r'<DataArray type="Float64" Name="([^"]+)" format="appended" offset="(\d+)" />', head):
+        off = int(m.group(2))
+        nbytes = int(np.frombuffer(blob[off:off + 8], dtype=np.uint64)[0])
+        out.append((m.group(1), np.frombuffer(blob[off + 8:off + 8 + nbytes], dtype=np.float64)))
+    return extent, out
+
+
+def check_vti(path, want_soa, ascii_):
+    """The file holds the interior cells of every variable, x fastest, in the reference's
+    variable order: bitwise in binary, as C++'s default 6-digit formatting in ascii."""
+    nv = want_soa.shape[0]
+    extent, arrays = parse_vti(path)
+    dims = want_soa.shape[1:][::-1]  # nx, ny[, nz]
+    assert extent == [0, dims[0], 0, dims[1], 0, dims[2] if len(dims) == 3 else 0]
+    assert [a[0] for a in arrays] == NAMES[:nv]
+    for v, (name, vals) in enumerate(arrays):
+        flat = want_soa[v].reshape(-1)
+        if ascii_:
+            assert vals == ["%g" % x for x in flat], name
+        else:
+            assert np.array_equal(vals, flat), name
+
+
+def write_outputs(tmp_path, ini_text, tag, ascii_):
+    d = tmp_path / tag
+    d.mkdir()
+    ini = tmp_path / f"{tag}.ini"
+    ini.write_text(ini_text.replace("nOutput=0", "nOutput=-1") + f"[output]\noutputDir={d}\noutputPrefix=o\n"
+                   + ("outputVtkAscii=true\n" if ascii_ else ""))
+    run_exe(ini, tmp_path)
+    files = sorted(p for p in os.listdir(d) if p.endswith(".vti"))
+    return d / files[0], d / files[-1]
+
+
+@pytest.mark.parametrize("ascii_", [False, True])
+def test_vti_hydro_2d_content(tmp_path, oracle, ascii_):
+    ini = INI_2D.replace("nStepmax=1000", "nStepmax=3")
+    first, last = write_outputs(tmp_path, ini, "h2", ascii_)
+    ekp = make_params("Hydro_Muscl_2D", 96, 64, xmax="1.5", gamma0="1.666", cfl="0.8",
+                      slope_type="2", riemann="hllc", bc=(1, 2, 3, 3, 1, 1))
+    op = oracle.from_ek(ekp)
+    f = lambda s: float(np.float32(s))
+    U = oracle.blast(op, radius=f("0.2"), center_x=f("0.6"), center_y=f("0.4"), center_z=0.5, density_in=1.0,
+                     density_out=f("1.2"), pressure_in=10.0, pressure_out=f("0.1"))
+    check_vti(first, problems.soa_from_reference(ekp, U), ascii_)
+    _, it, _ = oracle.run(op, U, 3, f("0.02"))
+    assert it == 3
+    check_vti(last, problems.soa_from_reference(ekp, U), ascii_)
+
+
+@pytest.mark.parametrize("ascii_", [False, True])
+def test_vti_hydro_3d_content(tmp_path, oracle, ascii_):
+    first, last = write_outputs(tmp_path, INI_3D, "h3", ascii_)
+    ekp = make_params("Hydro_Muscl_3D", 20, 16, 12, gamma0="1.4", cfl="0.5", slope_type="2", riemann="hllc",
+                      bc=(1, 2, 3, 3, 2, 1))
+    op = oracle.from_ek(ekp)
+    f = lambda s: float(np.float32(s))
+    U = oracle.blast(op, radius=f("0.3"), center_x=f("0.4"), center_y=0.5, center_z=f("0.55"), density_in=1.0,
+                     density_out=1.0, pressure_in=10.0, pressure_out=f("0.1"))
+    check_vti(first, problems.soa_from_reference(ekp, U), ascii_)
+    _, it, _ = oracle.run(op, U, 4, 1e30)
+    assert it == 4
+    check_vti(last, problems.soa_from_reference(ekp, U), ascii_)
+
+
+def test_vti_mhd_2d_content(tmp_path, oracle):
+    first, last = write_outputs(tmp_path, INI_MHD.replace("nStepmax=25", "nStepmax=5"), "m2", False)
+    ekp = make_params("MHD_Muscl_2D", 60, 44, gamma0="1.666", cfl="0.8", slope_type="2",
+                      riemann="hlld", smallr="1e-7", smallc="1e-7", bc=(3,) * 6)
+    op = oracle.from_ek(ekp)
+    U = oracle.orszag_tang(op)
+    check_vti(first, problems.soa_from_reference(ekp, U), False)
+    _, it, _ = oracle.mhd_run(op, U, 5, 1e30)
+    assert it == 5
+    check_vti(last, problems.soa_from_reference(ekp, U), False)
+
+
+def test_vti_mhd_3d_content(tmp_path, oracle):
+    first, last = write_outputs(tmp_path, INI_MHD3D, "m3", False)
+    ekp = make_params("MHD_Muscl_3D", 14, 12, 16, **MHD3D_EKP)
+    op = oracle.from_ek(ekp)
+    U = mhd3d_blast_oracle_ic(oracle, op)
+    check_vti(first, problems.soa_from_reference(ekp, U), False)
+    _, it, _ = oracle.mhd3d_run(op, U, 6, 1e30)
+    assert it == 6
+    check_vti(last, problems.soa_from_reference(ekp, U), False)
 
 
 # ---------------------------------------------------------------- HDF5 output and restart

@@ -112,3 +112,56 @@ def test_autotune_restores_state_and_keeps_results():
             assert abs(b[0] - ref[k][0][0].sum()) <= 1e-10 * ref[k][0][0].sum()
     finally:
         P.close()
+
+
+def test_finished_job_stays_frozen_while_others_step():
+    # a job that reached its nStepmax keeps its final time, iteration and state while the
+    # pool goes on stepping the others over several calls
+    jobs = jobs2d(native.EK_FP64)[:2]
+    ref = [single_run(jobs[0][0], jobs[0][1], 5), single_run(jobs[1][0], jobs[1][1], 20)]
+    P = pool.Pool(jobs[0][0], 256 << 20, 8)
+    try:
+        a = P.admit(jobs[0][0], jobs[0][1], 5)
+        b = P.admit(jobs[1][0], jobs[1][1], 20)
+        for n in (6, 1, 2):
+            P.step(n)
+            t, it, fin = P.status(a)
+            print(f"case=frozen job after step({n}): it={it} fin={fin}", flush=True)
+            assert fin and it == 5 and t == ref[0][1]
+            assert np.array_equal(P.download(a), ref[0][0])
+        P.step(12)
+        t, it, fin = P.status(b)
+        assert fin and it == 20 and t == ref[1][1]
+        assert np.array_equal(P.download(b), ref[1][0])
+        assert np.array_equal(P.download(a), ref[0][0])
+    finally:
+        P.close()
+
+
+@pytest.mark.parametrize("stage", [1, 2, 3])
+def test_failed_admission_leaves_the_pool_as_it_was(stage):
+    # an admission that fails after its arena range was reserved gives the range and the slot
+    # back; the jobs already admitted keep stepping to their single-grid results and the
+    # next admission succeeds
+    jobs = jobs2d(native.EK_FP64)
+    ref = [single_run(p, U, 12) for p, U in jobs]
+    P = pool.Pool(jobs[0][0], 256 << 20, 8)
+    try:
+        ids = [P.admit(p, U, 12) for p, U in jobs[:2]]
+        P.step(3)
+        free0, act0 = P.free_bytes(), P.active()
+        P.test_fail_admit(stage)
+        with pytest.raises(native.EkError):
+            P.admit(jobs[2][0], jobs[2][1], 12)
+        print(f"case=admission rollback stage {stage}: free {P.free_bytes()} / {free0}", flush=True)
+        assert P.free_bytes() == free0 and P.active() == act0
+        P.step(2)
+        P.test_fail_admit(0)
+        ids.append(P.admit(jobs[3][0], jobs[3][1], 12))
+        P.step(20)
+        for k, jid in zip((0, 1, 3), ids):
+            t, it, fin = P.status(jid)
+            assert fin and it == 12 and t == ref[k][1]
+            assert np.array_equal(P.download(jid), ref[k][0]), k
+    finally:
+        P.close()

@@ -117,3 +117,44 @@ def test_context_group_job_that_cannot_be_created_fails_alone():
         assert G.used == 0 and not G.jobs
     finally:
         G.close()
+
+
+class _FakeCtx:
+    """A stepping context: run() advances the iteration, or raises for a failing job."""
+
+    def __init__(self, fail):
+        self.fail = fail
+        self.closed = False
+
+    def run(self, n, nStepmax, tEnd, t, it):
+        if self.fail:
+            raise native.EkError("ek_run failed (2): injected HIP error")
+        it2 = min(nStepmax, it + n)
+        return t + 0.1 * (it2 - it), it2, 0.1
+
+    def download(self, layout=None):
+        return np.zeros(3)
+
+    def close(self):
+        self.closed = True
+
+
+def test_context_group_step_failure_fails_that_job_alone():
+    # one context's run raises during a step: that job alone is reported done and failed,
+    # the other jobs keep stepping to completion in the same and later steps
+    G = serving.ContextGroup(1 << 40)
+    G._create = staticmethod(lambda p, U: _FakeCtx(fail=(p == "bad")))
+    try:
+        G.job_bytes = lambda p: 1
+        ids = [G.admit(p, None, 4) for p in ("ok", "bad", "ok")]
+        for _ in range(6):
+            G.step(1)
+        for jid, p in zip(ids, ("ok", "bad", "ok")):
+            _, it, fin = G.status(jid)
+            assert fin
+            if p == "bad":
+                assert isinstance(G.failed(jid), native.EkError) and it == 0
+            else:
+                assert G.failed(jid) is None and it == 4
+    finally:
+        G.close()
