@@ -6,6 +6,7 @@ import this module.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -56,48 +57,70 @@ def lib():
     if _L is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        D = P(C.c_double)
-        L.eko_params_init.argtypes = [P(EkoParams)]
-        L.eko_compute_primitives.argtypes = [P(EkoParams), D, D, D]
-        L.eko_slope_scalar.argtypes = [P(EkoParams), C.c_double, C.c_double, C.c_double, D]
-        L.eko_trace_along_dir.argtypes = [P(EkoParams), D, D, D, D, C.c_double, C.c_double,
-                                          C.c_double, C.c_int, D]
-        L.eko_riemann_hydro.argtypes = [P(EkoParams), D, D, D]
-        L.eko_init_implode.argtypes = [P(EkoParams), P(EkoImplode), D]
-        L.eko_init_blast.argtypes = [P(EkoParams), P(EkoBlast), D]
-        L.eko_make_boundaries.argtypes = [P(EkoParams), D]
-        L.eko_compute_inv_dt.argtypes = [P(EkoParams), D]
-        L.eko_compute_inv_dt.restype = C.c_double
-        L.eko_godunov_unsplit_impl.argtypes = [P(EkoParams), D, D, D, D, D, D, C.c_double]
-        L.eko_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
-        L.eko_run.restype = C.c_int
-        L.eko_num_threads.restype = C.c_int
-        A8 = C.POINTER(C.c_double * 8)
-        L.eko_mhd_constoprim.argtypes = [P(EkoParams), D, D, D, D]
-        L.eko_mhd_speed_info.argtypes = [P(EkoParams), D, D]
-        L.eko_riemann_mhd.argtypes = [P(EkoParams), D, D, D]
-        L.eko_compute_emfz.argtypes = [P(EkoParams), A8]
-        L.eko_compute_emfz.restype = C.c_double
-        L.eko_mhd_trace_2d.argtypes = [P(EkoParams), D, D, C.c_double, C.c_double, D, D, D]
-        L.eko_mhd_init_orszag_tang.argtypes = [P(EkoParams), D]
-        L.eko_mhd_convert_to_primitives.argtypes = [P(EkoParams), D, D]
-        L.eko_mhd_inv_dt.argtypes = [P(EkoParams), D]
-        L.eko_mhd_inv_dt.restype = C.c_double
-        L.eko_mhd_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
-        L.eko_mhd_run.restype = C.c_int
-        L.eko_mhd3d_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
-        L.eko_mhd3d_run.restype = C.c_int
-        L.eko_mhd_run_dt.argtypes = [P(EkoParams), D, C.c_int, C.c_double]
-        L.eko_mhd3d_run_dt.argtypes = [P(EkoParams), D, C.c_int, C.c_double]
-        L.eko_mhd3d_convert_to_primitives.argtypes = [P(EkoParams), D, D]
-        L.eko_mhd3d_inv_dt.argtypes = [P(EkoParams), D]
-        L.eko_mhd3d_inv_dt.restype = C.c_double
-        L.eko_compute_emf_dir.argtypes = [P(EkoParams), A8, C.c_int]
-        L.eko_compute_emf_dir.restype = C.c_double
-        _L = L
+        _L = _bind(C.CDLL(LIB_PATH))
     return _L
+
+
+def load(path: str):
+    """A separately built copy of the oracle (e.g. one with a slip planted by a test), bound
+    like the oracle itself; make the module's functions use it with `using`."""
+    return _bind(C.CDLL(path))
+
+
+@contextlib.contextmanager
+def using(L):
+    """Within the block, every function of this module calls library L instead of the oracle."""
+    global _L
+    old = _L
+    _L = L
+    try:
+        yield L
+    finally:
+        _L = old
+
+
+def _bind(L):
+    """Sets the argument and result types of every oracle entry point on L."""
+    P = C.POINTER
+    D = P(C.c_double)
+    L.eko_params_init.argtypes = [P(EkoParams)]
+    L.eko_compute_primitives.argtypes = [P(EkoParams), D, D, D]
+    L.eko_slope_scalar.argtypes = [P(EkoParams), C.c_double, C.c_double, C.c_double, D]
+    L.eko_trace_along_dir.argtypes = [P(EkoParams), D, D, D, D, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, D]
+    L.eko_riemann_hydro.argtypes = [P(EkoParams), D, D, D]
+    L.eko_init_implode.argtypes = [P(EkoParams), P(EkoImplode), D]
+    L.eko_init_blast.argtypes = [P(EkoParams), P(EkoBlast), D]
+    L.eko_make_boundaries.argtypes = [P(EkoParams), D]
+    L.eko_compute_inv_dt.argtypes = [P(EkoParams), D]
+    L.eko_compute_inv_dt.restype = C.c_double
+    L.eko_godunov_unsplit_impl.argtypes = [P(EkoParams), D, D, D, D, D, D, C.c_double]
+    L.eko_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
+    L.eko_run.restype = C.c_int
+    L.eko_num_threads.restype = C.c_int
+    A8 = C.POINTER(C.c_double * 8)
+    L.eko_mhd_constoprim.argtypes = [P(EkoParams), D, D, D, D]
+    L.eko_mhd_speed_info.argtypes = [P(EkoParams), D, D]
+    L.eko_riemann_mhd.argtypes = [P(EkoParams), D, D, D]
+    L.eko_compute_emfz.argtypes = [P(EkoParams), A8]
+    L.eko_compute_emfz.restype = C.c_double
+    L.eko_mhd_trace_2d.argtypes = [P(EkoParams), D, D, C.c_double, C.c_double, D, D, D]
+    L.eko_mhd_init_orszag_tang.argtypes = [P(EkoParams), D]
+    L.eko_mhd_convert_to_primitives.argtypes = [P(EkoParams), D, D]
+    L.eko_mhd_inv_dt.argtypes = [P(EkoParams), D]
+    L.eko_mhd_inv_dt.restype = C.c_double
+    L.eko_mhd_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
+    L.eko_mhd_run.restype = C.c_int
+    L.eko_mhd3d_run.argtypes = [P(EkoParams), D, C.c_int, C.c_double, D, P(C.c_int), D]
+    L.eko_mhd3d_run.restype = C.c_int
+    L.eko_mhd_run_dt.argtypes = [P(EkoParams), D, C.c_int, C.c_double]
+    L.eko_mhd3d_run_dt.argtypes = [P(EkoParams), D, C.c_int, C.c_double]
+    L.eko_mhd3d_convert_to_primitives.argtypes = [P(EkoParams), D, D]
+    L.eko_mhd3d_inv_dt.argtypes = [P(EkoParams), D]
+    L.eko_mhd3d_inv_dt.restype = C.c_double
+    L.eko_compute_emf_dir.argtypes = [P(EkoParams), A8, C.c_int]
+    L.eko_compute_emf_dir.restype = C.c_double
+    return L
 
 
 def dp(a: np.ndarray):
