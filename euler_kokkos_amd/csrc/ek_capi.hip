@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -621,5 +622,139 @@ extern "C" int ek_device_info(int* cus, int* mem_clock_khz, int* bus_width_bits,
   if (name && name_len > 0) {
     std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
   }
+  return EK_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// ek_run_noalloc: the time loop of ek_run on caller-owned device state, for callers that
+// cannot allocate device memory. Everything the kernels need beyond the two state buffers
+// (the three time slots, the spread CFL words and, for 3D MHD, the padded work area) lives
+// in `scratch`, a fixed array of every device's code object. Its device address is taken
+// once per device with hipGetSymbolAddress; host code only ever uses that address.
+// Calls that overlap on one device share the array and race.
+
+namespace {
+constexpr size_t kScratchBytes = EK_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kScratchSlots = 0;                  // 3 TimeSlots (96 B), padded to 256
+constexpr size_t kScratchSpread = 256;               // SPREAD_ELEMS words (48 KiB)
+constexpr size_t kScratchWork = kScratchSpread + ((size_t)ek::SPREAD_ELEMS * 8 + 255) / 256 * 256;
+static_assert(3 * sizeof(ek::TimeSlot) <= kScratchSpread - kScratchSlots, "time slots fit");
+static_assert(kScratchWork < kScratchBytes, "room for a work area");
+constexpr int kMaxDevices = 64;
+} // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices];  // device address of `scratch` per device
+
+// Device address of this device's `scratch` (looked up on the first call per device).
+static int scratch_address(int dev, unsigned char** out)
+{
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(nullptr, EK_EINVAL, "device ordinal out of range for ek_run_noalloc");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(nullptr, EK_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return EK_OK;
+}
+
+namespace ek {
+// Clears the three time slots and their spread words and puts (t0, it0) in slot 0: a kernel,
+// not hipMemsetAsync, so that the call can also be recorded into a graph.
+__global__ __launch_bounds__(256) void noalloc_init_kernel(TimeSlot* ts, unsigned long long* spread, double t0,
+                                                           int it0)
+{
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < SPREAD_ELEMS; i += (long long)gridDim.x * 256)
+    spread[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x < 3) {
+    TimeSlot s{};
+    if (threadIdx.x == 0) {
+      s.t = t0;
+      s.it = it0;
+    }
+    ts[threadIdx.x] = s;
+  }
+}
+} // namespace ek
+
+extern "C" int ek_run_noalloc(const ek_params* params, void* U, void* U2, int nsteps, int nStepmax, double tEnd,
+                              double* t_io, int* it_io, double* dt_last, int* in_U2, void* stream)
+{
+  if (!params || !U || !U2 || !t_io || !it_io || !in_U2)
+    return fail(nullptr, EK_EINVAL, "null argument");
+  if (nsteps < 0)
+    return fail(nullptr, EK_EINVAL, "nsteps must be >= 0");
+  std::string why;
+  if (ek::validate_params(params, why))
+    return fail(nullptr, EK_EINVAL, why);
+  // a context on the host stack only: its device pointers are the caller's buffers and
+  // `scratch`, and nothing in it is freed
+  ek_ctx c;
+  c.p = *params;
+  c.st = (int)params->slope_type;
+  c.elem = params->precision == EK_FP64 ? 8 : 4;
+  ek::fill_geom<double>(c.p, c.g64, c.gas64);
+  ek::fill_geom<float>(c.p, c.g32, c.gas32);
+  c.ncell = c.g64.vstride;
+  c.cz = params->dim == 3 ? ek::hydro3d_z_chunk(params->nx, params->ny, params->nz) : 1;
+  c.stream = (hipStream_t)stream;
+  c.U[0] = U;
+  c.U[1] = U2;
+  if (c.p.mhd && c.p.dim == 3) {
+    const long long wb = ek::fp64::mhd3_scratch_elems(c.p.nx, c.p.ny, c.p.nz) * (long long)c.elem;
+    if ((double)wb > (double)(kScratchBytes - kScratchWork))
+      return fail(nullptr, EK_EINVAL,
+                  "grid too large for ek_run_noalloc: the 3D MHD work area needs " + std::to_string(wb) +
+                      " bytes and the fixed scratch holds " + std::to_string(kScratchBytes - kScratchWork) +
+                      " (use ek_create/ek_run)");
+  }
+  if (hipGetDevice(&c.device) != hipSuccess)
+    return fail(nullptr, EK_EHIP, "hipGetDevice failed");
+  unsigned char* base = nullptr;
+  if (int r = scratch_address(c.device, &base))
+    return r;
+  c.ts = (ek::TimeSlot*)(base + kScratchSlots);
+  c.spread = (unsigned long long*)(base + kScratchSpread);
+  c.work = (c.p.mhd && c.p.dim == 3) ? (void*)(base + kScratchWork) : nullptr;
+  auto hip_err = [](hipError_t e, const char* what) {
+    return fail(nullptr, EK_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  *in_U2 = 0;
+  if (nsteps == 0)
+    return EK_OK;
+  hipLaunchKernelGGL(ek::noalloc_init_kernel, dim3(24), dim3(256), 0, c.stream, c.ts, c.spread, *t_io, *it_io);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return hip_err(e, "time-slot initialisation");
+  if ((e = launch_inv_dt_into(&c, &c.ts[0].invdt_bits, c.spread)) != hipSuccess)
+    return hip_err(e, "initial CFL reduction");
+  for (int r = 0; r < nsteps; r++) {
+    ek::StepCtl ctl{};
+    ctl.ts = c.ts;
+    ctl.spread = c.spread;
+    ctl.slot = r % 3;
+    ctl.mode = 0;
+    ctl.cfl = c.p.cfl;
+    ctl.tEnd = tEnd;
+    ctl.nStepmax = nStepmax;
+    if ((e = launch_one(&c, ctl, c.U[r % 2], c.U[(r + 1) % 2])) != hipSuccess)
+      return hip_err(e, "step launch");
+  }
+  ek::TimeSlot fin;
+  if ((e = hipMemcpyAsync(&fin, &c.ts[nsteps % 3], sizeof(fin), hipMemcpyDeviceToHost, c.stream)) != hipSuccess ||
+      (e = hipStreamSynchronize(c.stream)) != hipSuccess)
+    return hip_err(e, "reading the final time slot");
+  *in_U2 = (fin.it - *it_io) % 2;
+  *t_io = fin.t;
+  *it_io = fin.it;
+  if (dt_last)
+    *dt_last = fin.dt;
   return EK_OK;
 }

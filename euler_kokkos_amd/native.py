@@ -21,7 +21,9 @@ RIEMANN = {"approx": 0, "llf": 1, "hll": 2, "hllc": 3, "hlld": 4}
 # Exported symbols declared by include/ek_hip.h (checked by the CPU test-suite).
 EXPORTS = ("ek_create", "ek_destroy", "ek_last_error", "ek_upload", "ek_download", "ek_inv_dt",
            "ek_step", "ek_run", "ek_budgets", "ek_sync", "ek_device_bytes", "ek_last_run_time",
-           "ek_host_alloc", "ek_host_free", "ek_device_info")
+           "ek_host_alloc", "ek_host_free", "ek_device_info", "ek_run_noalloc")
+# size of the fixed device array behind ek_run_noalloc (EK_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 16 << 20
 
 
 class EkParams(C.Structure):
@@ -73,8 +75,10 @@ def lib() -> C.CDLL:
     L.ek_host_alloc.argtypes = [C.c_size_t, P(C.c_void_p)]
     L.ek_host_free.argtypes = [C.c_void_p]
     L.ek_device_info.argtypes = [P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_int), C.c_char_p, C.c_int]
+    L.ek_run_noalloc.argtypes = [P(EkParams), C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                 P(C.c_double), P(C.c_int), P(C.c_double), P(C.c_int), C.c_void_p]
     for n in ("ek_upload", "ek_download", "ek_inv_dt", "ek_step", "ek_run", "ek_budgets", "ek_sync",
-              "ek_host_alloc", "ek_host_free", "ek_device_info"):
+              "ek_host_alloc", "ek_host_free", "ek_device_info", "ek_run_noalloc"):
         getattr(L, n).restype = C.c_int
     _LIB = L
     return L
@@ -90,6 +94,19 @@ def device_info() -> dict:
         raise EkError(f"ek_device_info failed ({rc})")
     return {"cus": cus.value, "mem_clock_khz": mclk.value, "bus_width_bits": bw.value,
             "sclk_khz": sclk.value, "name": name.value.decode(errors="replace")}
+
+
+def run_noalloc(params: EkParams, U: int, U2: int, nsteps: int, nStepmax: int, tEnd: float,
+                t: float = 0.0, it: int = 0, stream: int = 0):
+    """ek_run_noalloc on caller-owned device buffers (device addresses U, U2 holding SoA state
+    in the params' precision): returns (t, it, dt_last, in_U2), in_U2 saying which buffer
+    holds the final state."""
+    t_c, it_c, dt_c, in2 = C.c_double(t), C.c_int(it), C.c_double(0.0), C.c_int(0)
+    rc = lib().ek_run_noalloc(C.byref(params), C.c_void_p(U), C.c_void_p(U2), nsteps, nStepmax, tEnd,
+                              C.byref(t_c), C.byref(it_c), C.byref(dt_c), C.byref(in2), C.c_void_p(stream))
+    if rc != 0:
+        raise EkError(f"ek_run_noalloc failed ({rc}): {lib().ek_last_error(None).decode()}")
+    return t_c.value, it_c.value, dt_c.value, in2.value
 
 
 class _Pinned:

@@ -103,6 +103,28 @@ int ek_step(ek_ctx* ctx, double dt);
 int ek_run(ek_ctx* ctx, int nsteps, int nStepmax, double tEnd, double* t_io, int* it_io,
            double* dt_last);
 
+/* Size of the fixed device array behind ek_run_noalloc (bytes, per device). */
+#define EK_NOALLOC_SCRATCH_BYTES (16u << 20)
+
+/* The time loop of ek_run on caller-owned device state, allocating nothing. No reference
+ * counterpart: it is for callers that cannot allocate device memory (code recorded under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ *
+ * U holds the state on entry in EK_LAYOUT_SOA order (interior cells only,
+ * U[var][k][j][i]) in the params' precision (double for EK_FP64, float for EK_FP32); U2 is
+ * a second device buffer of the same size. Steps alternate between the two; on return
+ * *in_U2 is 1 when the final state is in U2, else 0. t_io / it_io / dt_last / nStepmax /
+ * tEnd mean what they mean for ek_run. Kernels are launched on `stream` (a hipStream_t, NULL
+ * for the default stream), one launch per step (no graph), and the call synchronizes with
+ * that stream to read the final time. The time slots, CFL words and, for 3D MHD, the padded
+ * work area live in a fixed device array of EK_NOALLOC_SCRATCH_BYTES per device, held from
+ * the library's first use on that device until the process ends: a 3D MHD grid whose work
+ * area (78 reals per ghost-padded cell) does not fit is rejected with EK_EINVAL. Every call
+ * on one device uses the same array, so calls that overlap on one device race; each call
+ * clears what it needs first. Errors are reported through ek_last_error(NULL). */
+int ek_run_noalloc(const ek_params* params, void* U, void* U2, int nsteps, int nStepmax, double tEnd,
+                   double* t_io, int* it_io, double* dt_last, int* in_U2, void* stream);
+
 /* Conservation budgets: Neumaier-compensated FP64 sums over the interior of every
  * conserved variable (rho, E, rho*u, rho*v[, rho*w, Bx, By, Bz]) into sums[0..nbvar-1].
  * Synchronizes. (No reference counterpart; used for round-off conservation checks.) */

@@ -72,3 +72,23 @@ def test_create_rejects_invalid_before_allocating(case):
     msg = native.lib().ek_last_error(None).decode()
     assert want in msg, msg
 
+
+
+def test_noalloc_rejects_on_the_host():
+    """ek_run_noalloc refuses invalid calls before touching a device: null buffers, an invalid
+    configuration, and a 3D MHD grid whose work area does not fit its fixed scratch (78 reals
+    per ghost-padded cell: a 23^3 FP64 grid fits, 24^3 does not)."""
+    ok = make_params("Hydro_Muscl_3D", 16, 16, 16, riemann="hllc")
+    with pytest.raises(native.EkError, match="null argument"):
+        native.run_noalloc(ok, 0, 0, 1, 1, 1e30)
+    with pytest.raises(native.EkError, match="nsteps"):
+        native.run_noalloc(ok, 1, 2, -1, 1, 1e30)
+    bad = make_params("Hydro_Muscl_3D", 16, 16, 16, riemann="hll")
+    with pytest.raises(native.EkError, match="riemann"):
+        native.run_noalloc(bad, 1, 2, 1, 1, 1e30)
+    for n, fits in ((23, True), (24, False)):
+        mhd = make_params("MHD_Muscl_3D", n, n, n, gamma0="1.6666667", riemann="hlld", bc=(3,) * 6)
+        assert ((n + 6) ** 3 * 78 * 8 <= native.NOALLOC_SCRATCH_BYTES - 49408) == fits
+        if not fits:
+            with pytest.raises(native.EkError, match="too large for ek_run_noalloc"):
+                native.run_noalloc(mhd, 1, 2, 1, 1, 1e30)
