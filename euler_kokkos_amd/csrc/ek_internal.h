@@ -11,6 +11,9 @@
 #include "ek_hip.h"
 
 namespace ek {
+template <class T>
+struct PoolJobM;  // 3D MHD job record of the job pool (ek_pool_mhd3d.h)
+
 // Checks an ek_params against what this library provides (ek_capi.hip); 0 when valid.
 int validate_params(const ek_params* p, std::string& why);
 // FP32 copy of n FP64 values already on the device (round to nearest, denormals kept, as
@@ -48,6 +51,12 @@ hipError_t mhd3_step(int rs, int st, const double* Uin, double* Uout, double* sc
 hipError_t mhd3_inv_dt(const double* U, const Geom<double>& geo, const Gas<double>& gas, unsigned long long* dst,
                        unsigned long long* spread, hipStream_t s);
 long long mhd3_scratch_elems(int nx, int ny, int nz);
+// 3D MHD jobs of the job pool (ek_mhd3d_fp64.hip): the five pipeline kernels over the work
+// lists wp (padded-grid chunks) and wu (interior chunks); grid[5] from pool_mhd3_grids
+hipError_t pool_mhd3_step(int rs, int st, const double* Ua, double* Ub, double* W, const PoolJobM<double>* jobs,
+                          const PoolWork* wp, int np, const PoolWork* wu, int nu, TimeSlot* ts, int slot,
+                          const int* grid, hipStream_t s);
+void pool_mhd3_grids(int rs, int st, int cus, int* grid);
 } // namespace fp64
 namespace fp32 {
 hipError_t step(int dim, int rs, int st, const float* Uin, float* Uout, const Geom<float>& geo,
@@ -76,5 +85,11 @@ hipError_t mhd3_step(int rs, int st, const float* Uin, float* Uout, float* scrat
 hipError_t mhd3_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>& gas, unsigned long long* dst,
                        unsigned long long* spread, hipStream_t s);
 long long mhd3_scratch_elems(int nx, int ny, int nz);
+// 3D MHD jobs of the job pool (ek_mhd3d_fp32.hip): the five pipeline kernels over the work
+// lists wp (padded-grid chunks) and wu (interior chunks); grid[5] from pool_mhd3_grids
+hipError_t pool_mhd3_step(int rs, int st, const float* Ua, float* Ub, float* W, const PoolJobM<float>* jobs,
+                          const PoolWork* wp, int np, const PoolWork* wu, int nu, TimeSlot* ts, int slot,
+                          const int* grid, hipStream_t s);
+void pool_mhd3_grids(int rs, int st, int cus, int* grid);
 } // namespace fp32
 } // namespace ek

@@ -131,15 +131,9 @@ __device__ __forceinline__ bool step_dt(const StepCtl& c, double& dt)
 // (ConvertToPrimitivesFunctor3D_MHD), whose right / top / front face fields are the padded
 // values of those neighbours, read from the interior state through the same map.
 template <class T>
-__global__ __launch_bounds__(256) void padprims_kernel(const T* __restrict__ U, T* __restrict__ S, Work3 w,
-                                                       Geom<T> geo, Gas<T> gas, StepCtl ctl)
+__device__ __forceinline__ void padprims_cell(const T* __restrict__ U, T* __restrict__ S, const Work3& w,
+                                              const Geom<T>& geo, const Gas<T>& gas, int i, int j, int k)
 {
-  double dt;
-  if (!step_dt(ctl, dt))
-    return;
-  int i, j, k;
-  if (!cell_of<T>(w, i, j, k))
-    return;
   bool fx, fy, fz;
   const int si = bc_map(i - 3, w.nx, geo.bc[0], geo.bc[1], fx);
   const int sj = bc_map(j - 3, w.ny, geo.bc[2], geo.bc[3], fy);
@@ -180,15 +174,24 @@ __global__ __launch_bounds__(256) void padprims_kernel(const T* __restrict__ U, 
     S[(A_Q + v) * PN + po] = q[v];
 }
 
-// edge electric fields and transverse face-field slopes on 1 <= i,j,k < size-1
-template <class T, int ST>
-__global__ __launch_bounds__(256) void efield_kernel(T* __restrict__ S, Work3 w, StepCtl ctl)
+template <class T>
+__global__ __launch_bounds__(256) void padprims_kernel(const T* __restrict__ U, T* __restrict__ S, Work3 w,
+                                                       Geom<T> geo, Gas<T> gas, StepCtl ctl)
 {
   double dt;
   if (!step_dt(ctl, dt))
     return;
   int i, j, k;
-  if (!cell_of<T>(w, i, j, k) || i < 1 || j < 1 || k < 1 || i >= w.PX - 1 || j >= w.PY - 1 || k >= w.PZ - 1)
+  if (!cell_of<T>(w, i, j, k))
+    return;
+  padprims_cell<T>(U, S, w, geo, gas, i, j, k);
+}
+
+// edge electric fields and transverse face-field slopes on 1 <= i,j,k < size-1
+template <class T, int ST>
+__device__ __forceinline__ void efield_cell(T* __restrict__ S, const Work3& w, int i, int j, int k)
+{
+  if (i < 1 || j < 1 || k < 1 || i >= w.PX - 1 || j >= w.PY - 1 || k >= w.PZ - 1)
     return;
   const long long PN = w.PN, o = pidx(w, i, j, k);
   const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
@@ -221,15 +224,24 @@ __global__ __launch_bounds__(256) void efield_kernel(T* __restrict__ S, Work3 w,
   S[(A_DB + 5) * PN + o] = slope1<T, ST>(u(IC, 0), u(IC, sy), u(IC, -sy));
 }
 
-// the 3D trace (trace_unsplit_mhd_3d_simpler) on 1 <= i,j,k < size-2
 template <class T, int ST>
-__global__ __launch_bounds__(256) void trace_kernel(T* __restrict__ S, Work3 w, Geom<T> geo, Gas<T> gas, StepCtl ctl)
+__global__ __launch_bounds__(256) void efield_kernel(T* __restrict__ S, Work3 w, StepCtl ctl)
 {
   double dt;
   if (!step_dt(ctl, dt))
     return;
   int i, j, k;
-  if (!cell_of<T>(w, i, j, k) || i < 1 || j < 1 || k < 1 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
+  if (!cell_of<T>(w, i, j, k))
+    return;
+  efield_cell<T, ST>(S, w, i, j, k);
+}
+
+// the 3D trace (trace_unsplit_mhd_3d_simpler) on 1 <= i,j,k < size-2
+template <class T, int ST>
+__device__ __forceinline__ void trace_cell(T* __restrict__ S, const Work3& w, const Geom<T>& geo, const Gas<T>& gas,
+                                           double dt, int i, int j, int k)
+{
+  if (i < 1 || j < 1 || k < 1 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
     return;
   const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
   const long long PN = w.PN, o = pidx(w, i, j, k);
@@ -336,6 +348,18 @@ __global__ __launch_bounds__(256) void trace_kernel(T* __restrict__ S, Work3 w, 
   put(K_Z + 6, dBz);
 }
 
+template <class T, int ST>
+__global__ __launch_bounds__(256) void trace_kernel(T* __restrict__ S, Work3 w, Geom<T> geo, Gas<T> gas, StepCtl ctl)
+{
+  double dt;
+  if (!step_dt(ctl, dt))
+    return;
+  int i, j, k;
+  if (!cell_of<T>(w, i, j, k))
+    return;
+  trace_cell<T, ST>(S, w, geo, gas, dt, i, j, k);
+}
+
 template <class T>
 struct RecView {
   const T* S;
@@ -428,17 +452,11 @@ __device__ __forceinline__ Edge<T> edge_state(const RecView<T>& R, unsigned cb, 
 // Launch 32 * ceil(nchunk / 8) blocks: block b takes task (b / 8) % 4 of chunk
 // (b / 32) * 8 + b % 8, so the four tasks of a chunk run on one XCD (blocks are dealt to
 // the XCDs round robin) and share its L2 copy of the chunk's records.
+// One task of padded cell c: 0 its three Riemann problems, 1..3 its EMF z, y or x.
 template <class T, int RS>
-__global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w, Gas<T> gas, StepCtl ctl)
+__device__ __forceinline__ void fluxemf_cell(T* __restrict__ S, const Work3& w, const Gas<T>& gas, unsigned c,
+                                             int task)
 {
-  double dt;
-  if (!step_dt(ctl, dt))
-    return;
-  const unsigned b = blockIdx.x;
-  const int task = (b >> 3) & 3;
-  const unsigned c = (((b >> 5) << 3) | (b & 7)) * 256u + threadIdx.x;
-  if (c >= (unsigned long long)w.PN)
-    return;
   int i, j, k;
   split3(c, (unsigned)w.PX, (unsigned)w.PY, i, j, k);
   if (i < 3 || j < 3 || k < 3 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
@@ -502,6 +520,20 @@ __global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w
   }
 }
 
+template <class T, int RS>
+__global__ __launch_bounds__(256) void fluxemf_kernel(T* __restrict__ S, Work3 w, Gas<T> gas, StepCtl ctl)
+{
+  double dt;
+  if (!step_dt(ctl, dt))
+    return;
+  const unsigned b = blockIdx.x;
+  const int task = (b >> 3) & 3;
+  const unsigned c = (((b >> 5) << 3) | (b & 7)) * 256u + threadIdx.x;
+  if (c >= (unsigned long long)w.PN)
+    return;
+  fluxemf_cell<T, RS>(S, w, gas, c, task);
+}
+
 // CFL term of a 3D cell from its new state and new right/top/front face fields
 template <class T>
 __device__ __forceinline__ double cfl3(const Gas<T>& gas, const T* u, T mx, T my, T mz, const Geom<T>& geo)
@@ -541,27 +573,15 @@ __device__ __forceinline__ double cfl3(const Gas<T>& gas, const T* u, T mx, T my
   }
 }
 
-// update on the interior (UpdateFunctor3D_MHD + UpdateEmfFunctor3D) into the SoA output.
-// A grid-stride loop over a bounded grid (MHD3_UPDATE_BLOCKS): each block ends with one
-// atomic max on the time slot, and one short block per 256 cells (27k blocks at 192^3)
-// made those atomics cost three quarters of the kernel's time.
-constexpr unsigned MHD3_UPDATE_BLOCKS = 2048;
-
+// Flux update, CT update and the CFL term of the new state of interior cell c
+// (UpdateFunctor3D_MHD + UpdateEmfFunctor3D + ComputeDtFunctor3D_MHD on the result).
 template <class T>
-__global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T* __restrict__ Uout, Work3 w,
-                                                     Geom<T> geo, Gas<T> gas, StepCtl ctl)
+__device__ __forceinline__ double update_cell(const T* __restrict__ S, T* __restrict__ Uout, const Work3& w,
+                                              const Geom<T>& geo, const Gas<T>& gas, T dtdx, T dtdy, T dtdz,
+                                              long long c)
 {
-  __shared__ double sred[4];
-  double dt;
-  TimeSlot cur;
-  const bool first_block = blockIdx.x == 0;
-  if (!step_begin(ctl, first_block, dt, cur))
-    return;
-  const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
   const long long n = geo.vstride;
-  double local = 0.0;
-  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
-       c += (long long)gridDim.x * blockDim.x) {
+  {
     int ii, jj, kk;
     split3((unsigned)c, (unsigned)w.nx, (unsigned)w.ny, ii, jj, kk);
     const long long PN = w.PN;
@@ -625,7 +645,32 @@ __global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T*
     } else {
       cf = geo.bc[5] == BC_DIRICHLET ? u[IC] * T(-1) : u[IC];
     }
-    const double sv = cfl3<T>(gas, u, ar, bt, cf, geo);
+    return cfl3<T>(gas, u, ar, bt, cf, geo);
+  }
+}
+
+// update on the interior (UpdateFunctor3D_MHD + UpdateEmfFunctor3D) into the SoA output.
+// A grid-stride loop over a bounded grid (MHD3_UPDATE_BLOCKS): each block ends with one
+// atomic max on the time slot, and one short block per 256 cells (27k blocks at 192^3)
+// made those atomics cost three quarters of the kernel's time.
+constexpr unsigned MHD3_UPDATE_BLOCKS = 2048;
+
+template <class T>
+__global__ __launch_bounds__(256) void update_kernel(const T* __restrict__ S, T* __restrict__ Uout, Work3 w,
+                                                     Geom<T> geo, Gas<T> gas, StepCtl ctl)
+{
+  __shared__ double sred[4];
+  double dt;
+  TimeSlot cur;
+  const bool first_block = blockIdx.x == 0;
+  if (!step_begin(ctl, first_block, dt, cur))
+    return;
+  const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
+  const long long n = geo.vstride;
+  double local = 0.0;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (long long)gridDim.x * blockDim.x) {
+    const double sv = update_cell<T>(S, Uout, w, geo, gas, dtdx, dtdy, dtdz, c);
     local = (sv > local || !(sv == sv)) ? sv : local;
   }
   block_max_to<256>(invdt_word(ctl, (ctl.slot + 1) % 3, blockIdx.x), local, sred);

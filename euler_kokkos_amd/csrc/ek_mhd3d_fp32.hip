@@ -1,6 +1,7 @@
 // ek_mhd3d_fp32.hip -- 3D MHD step pipeline, float instantiation.
 #include "ek_internal.h"
 #include "ek_mhd3d.h"
+#include "ek_pool_mhd3d.h"
 
 namespace ek {
 namespace fp32 {
@@ -56,6 +57,18 @@ hipError_t mhd3_inv_dt(const float* U, const Geom<float>& geo, const Gas<float>&
 long long mhd3_scratch_elems(int nx, int ny, int nz)
 {
   return (long long)mhd3::A_TOTAL * (nx + 6) * (ny + 6) * (long long)(nz + 6);
+}
+
+hipError_t pool_mhd3_step(int rs, int st, const float* Ua, float* Ub, float* W, const PoolJobM<float>* jobs,
+                          const PoolWork* wp, int np, const PoolWork* wu, int nu, TimeSlot* ts, int slot,
+                          const int* grid, hipStream_t s)
+{
+  return pool::mhd3_step<float>(rs, st, Ua, Ub, W, jobs, wp, np, wu, nu, ts, slot, grid, s);
+}
+
+void pool_mhd3_grids(int rs, int st, int cus, int* grid)
+{
+  pool::mhd3_grids_for<float>(rs, st, cus, grid);
 }
 
 } // namespace fp32

@@ -1,6 +1,7 @@
 // ek_mhd3d_fp64.hip -- 3D MHD step pipeline, double instantiation.
 #include "ek_internal.h"
 #include "ek_mhd3d.h"
+#include "ek_pool_mhd3d.h"
 
 namespace ek {
 namespace fp64 {
@@ -56,6 +57,18 @@ hipError_t mhd3_inv_dt(const double* U, const Geom<double>& geo, const Gas<doubl
 long long mhd3_scratch_elems(int nx, int ny, int nz)
 {
   return (long long)mhd3::A_TOTAL * (nx + 6) * (ny + 6) * (long long)(nz + 6);
+}
+
+hipError_t pool_mhd3_step(int rs, int st, const double* Ua, double* Ub, double* W, const PoolJobM<double>* jobs,
+                          const PoolWork* wp, int np, const PoolWork* wu, int nu, TimeSlot* ts, int slot,
+                          const int* grid, hipStream_t s)
+{
+  return pool::mhd3_step<double>(rs, st, Ua, Ub, W, jobs, wp, np, wu, nu, ts, slot, grid, s);
+}
+
+void pool_mhd3_grids(int rs, int st, int cus, int* grid)
+{
+  pool::mhd3_grids_for<double>(rs, st, cus, grid);
 }
 
 } // namespace fp64

@@ -11,6 +11,7 @@
 #include "ek_geom_host.h"
 #include "ek_internal.h"
 #include "ek_pool_launch.h"
+#include "ek_pool_mhd3d.h"
 
 namespace {
 
@@ -18,6 +19,7 @@ struct HostJob {
   bool used = false;
   ek_params p{};
   long long off = 0, nelem = 0, alloc = 0;
+  long long woff = 0, walloc = 0; // 3D MHD: the job's padded work area in the work buffer
   long long adm_step = 0;
   int nStepmax = 0;
   double tEnd = 0.0;
@@ -30,6 +32,9 @@ struct HostJob {
 constexpr long long kAlign = 64; // elements
 constexpr int kGraphSteps = 6;   // lcm of the 2 arena buffers and the 3 time slots
 constexpr int kCz[4] = { 8, 16, 32, 64 };
+// 3D MHD pools: share of the arena for the two state buffers; the rest is the work buffer
+// (78 reals per ghost-padded cell against 2 x 8 per cell: 13-16 % for edges 64-192)
+constexpr double kMhdStateShare = 0.16;
 
 long long round_up(long long n)
 {
@@ -66,6 +71,15 @@ struct ek_pool {
   double* stage = nullptr;
   long long stage_elems = 0;
   double* partial = nullptr;
+  // 3D MHD kind: the jobs' work areas (first-fit like the arena) and the update's work list
+  bool mhd3 = false;
+  void* W = nullptr;
+  long long wcap = 0;
+  std::map<long long, long long> wfree;
+  ek::PoolWork* dworkU = nullptr;
+  int nworkU = 0;
+  int workU_cap = 0;
+  int mgrid[5] = { 1, 1, 1, 1, 1 };
   int test_fail_admit = 0; // ek_pool_test_fail_admit: 0 off, else the admission step that fails
 };
 
@@ -118,8 +132,49 @@ static ek::PoolJob<T> make_record(const ek_pool* p, int id)
   return r;
 }
 
+// 3D MHD record of job `id`
+template <class T>
+static ek::PoolJobM<T> make_record_m(const ek_pool* p, int id)
+{
+  const HostJob& h = p->jobs[id];
+  ek::PoolJobM<T> r{};
+  r.off = h.off;
+  r.woff = h.woff;
+  r.slot0 = 3 * id;
+  r.nStepmax = h.nStepmax;
+  r.cfl = h.p.cfl;
+  r.tEnd = h.tEnd;
+  r.w.nx = h.p.nx;
+  r.w.ny = h.p.ny;
+  r.w.nz = h.p.nz;
+  r.w.PX = h.p.nx + 6;
+  r.w.PY = h.p.ny + 6;
+  r.w.PZ = h.p.nz + 6;
+  r.w.PN = (long long)r.w.PX * r.w.PY * r.w.PZ;
+  ek::fill_geom<T>(h.p, r.geo, r.gas);
+  return r;
+}
+
+static size_t record_bytes(const ek_pool* p)
+{
+  if (p->mhd3)
+    return p->fp64 ? sizeof(ek::PoolJobM<double>) : sizeof(ek::PoolJobM<float>);
+  return p->fp64 ? sizeof(ek::PoolJob<double>) : sizeof(ek::PoolJob<float>);
+}
+
 static int upload_record(ek_pool* p, int id)
 {
+  if (p->mhd3) {
+    if (p->fp64) {
+      const ek::PoolJobM<double> r = make_record_m<double>(p, id);
+      PHIP(p, hipMemcpyAsync((ek::PoolJobM<double>*)p->djobs + id, &r, sizeof(r), hipMemcpyHostToDevice, p->stream));
+    } else {
+      const ek::PoolJobM<float> r = make_record_m<float>(p, id);
+      PHIP(p, hipMemcpyAsync((ek::PoolJobM<float>*)p->djobs + id, &r, sizeof(r), hipMemcpyHostToDevice, p->stream));
+    }
+    PHIP(p, hipStreamSynchronize(p->stream));
+    return EK_OK;
+  }
   if (p->fp64) {
     const ek::PoolJob<double> r = make_record<double>(p, id);
     PHIP(p, hipMemcpyAsync((ek::PoolJob<double>*)p->djobs + id, &r, sizeof(r), hipMemcpyHostToDevice, p->stream));
@@ -145,8 +200,56 @@ static void tiles_of(const ek_pool* p, const ek_params& j, int& ntx, int& nty, i
   }
 }
 
+// Device copy of a host work list into *buf (capacity *cap items, grown as needed); on
+// failure the previous list stays valid.
+static int put_list(ek_pool* p, const std::vector<ek::PoolWork>& w, ek::PoolWork** buf, int* cap)
+{
+  if ((int)w.size() > *cap) {
+    const int nc = (int)std::max<size_t>(w.size() * 2, 1024);
+    ek::PoolWork* nw = nullptr;
+    PHIP(p, hipMalloc(&nw, (size_t)nc * sizeof(ek::PoolWork)));
+    PHIP(p, hipStreamSynchronize(p->stream)); // no launch still reads the old list
+    hipFree(*buf);
+    *buf = nw;
+    *cap = nc;
+  }
+  if (!w.empty())
+    PHIP(p, hipMemcpyAsync(*buf, w.data(), w.size() * sizeof(ek::PoolWork), hipMemcpyHostToDevice, p->stream));
+  PHIP(p, hipStreamSynchronize(p->stream));
+  return EK_OK;
+}
+
+// 3D MHD work lists: (job, 256-cell chunk of its padded grid) for the four padded-grid
+// kernels, (job, 4096-cell chunk of its interior) for the update, the job's first chunk
+// flagged for its time bookkeeping
+static int rebuild_work_mhd3(ek_pool* p)
+{
+  std::vector<ek::PoolWork> wp, wu;
+  for (int id = 0; id < (int)p->jobs.size(); id++) {
+    const HostJob& h = p->jobs[id];
+    if (!h.used || h.finished)
+      continue;
+    const long long pn = (long long)(h.p.nx + 6) * (h.p.ny + 6) * (long long)(h.p.nz + 6);
+    const long long np = (pn + ek::MP_CHUNK - 1) / ek::MP_CHUNK, nu = (h.ncell + ek::MU_CHUNK - 1) / ek::MU_CHUNK;
+    for (long long c = 0; c < np; c++)
+      wp.push_back(ek::PoolWork{ id, (int)c, 0, 0 });
+    for (long long c = 0; c < nu; c++)
+      wu.push_back(ek::PoolWork{ id, (int)c, 0, c == 0 ? 1 : 0 });
+  }
+  if (int rc = put_list(p, wp, &p->dwork, &p->work_cap))
+    return rc;
+  if (int rc = put_list(p, wu, &p->dworkU, &p->workU_cap))
+    return rc;
+  p->nwork = (int)wp.size();
+  p->nworkU = (int)wu.size();
+  p->graph_ok = false;
+  return EK_OK;
+}
+
 static int rebuild_work(ek_pool* p)
 {
+  if (p->mhd3)
+    return rebuild_work_mhd3(p);
   std::vector<ek::PoolWork> w;
   for (int id = 0; id < (int)p->jobs.size(); id++) {
     const HostJob& h = p->jobs[id];
@@ -179,6 +282,15 @@ static int rebuild_work(ek_pool* p)
 
 static void set_grid(ek_pool* p)
 {
+  if (p->mhd3) {
+    const int rs = p->kind.riemann, st = (int)p->kind.slope_type;
+    if (p->fp64)
+      ek::fp64::pool_mhd3_grids(rs, st, p->cus, p->mgrid);
+    else
+      ek::fp32::pool_mhd3_grids(rs, st, p->cus, p->mgrid);
+    p->grid = p->mgrid[0];
+    return;
+  }
   const int occ = p->fp64 ? ek::fp64::pool_occupancy(kind_dim(p), p->variant, (int)p->kind.slope_type)
                           : ek::fp32::pool_occupancy(kind_dim(p), p->variant, (int)p->kind.slope_type);
   p->grid = p->cus * occ;
@@ -188,6 +300,16 @@ static hipError_t launch_one(ek_pool* p, long long n)
 {
   const int par = (int)(n % 2), slot = (int)(n % 3);
   const int st = (int)p->kind.slope_type;
+  if (p->mhd3) {
+    const int rs = p->kind.riemann;
+    if (p->fp64)
+      return ek::fp64::pool_mhd3_step(rs, st, (const double*)p->A[par], (double*)p->A[1 - par], (double*)p->W,
+                                      (const ek::PoolJobM<double>*)p->djobs, p->dwork, p->nwork, p->dworkU,
+                                      p->nworkU, p->ts, slot, p->mgrid, p->stream);
+    return ek::fp32::pool_mhd3_step(rs, st, (const float*)p->A[par], (float*)p->A[1 - par], (float*)p->W,
+                                    (const ek::PoolJobM<float>*)p->djobs, p->dwork, p->nwork, p->dworkU, p->nworkU,
+                                    p->ts, slot, p->mgrid, p->stream);
+  }
   if (p->fp64)
     return ek::fp64::pool_step(kind_dim(p), p->variant, st, (const double*)p->A[par], (double*)p->A[1 - par],
                                (const ek::PoolJob<double>*)p->djobs, p->dwork, p->nwork, p->ts, slot, p->grid,
@@ -249,35 +371,52 @@ static int refresh_status(ek_pool* p, bool& changed)
 
 // arena ------------------------------------------------------------------------------
 
-static long long arena_alloc(ek_pool* p, long long n)
+// first fit in a free list (offset -> length, elements)
+static long long list_alloc(std::map<long long, long long>& fl, long long n)
 {
-  for (auto it = p->freel.begin(); it != p->freel.end(); ++it) {
+  for (auto it = fl.begin(); it != fl.end(); ++it) {
     if (it->second >= n) {
       const long long off = it->first, len = it->second;
-      p->freel.erase(it);
+      fl.erase(it);
       if (len > n)
-        p->freel[off + n] = len - n;
+        fl[off + n] = len - n;
       return off;
     }
   }
   return -1;
 }
 
-static void arena_free(ek_pool* p, long long off, long long n)
+static void list_free(std::map<long long, long long>& fl, long long off, long long n)
 {
-  auto it = p->freel.emplace(off, n).first;
+  auto it = fl.emplace(off, n).first;
   auto nx = std::next(it);
-  if (nx != p->freel.end() && it->first + it->second == nx->first) {
+  if (nx != fl.end() && it->first + it->second == nx->first) {
     it->second += nx->second;
-    p->freel.erase(nx);
+    fl.erase(nx);
   }
-  if (it != p->freel.begin()) {
+  if (it != fl.begin()) {
     auto pv = std::prev(it);
     if (pv->first + pv->second == it->first) {
       pv->second += it->second;
-      p->freel.erase(it);
+      fl.erase(it);
     }
   }
+}
+
+static long long arena_alloc(ek_pool* p, long long n)
+{
+  return list_alloc(p->freel, n);
+}
+
+static void arena_free(ek_pool* p, long long off, long long n)
+{
+  list_free(p->freel, off, n);
+}
+
+// elements of a 3D MHD job's padded work area
+static long long work_elems(const ek_params& j)
+{
+  return ek::fp64::mhd3_scratch_elems(j.nx, j.ny, j.nz);
 }
 
 static int ensure_stage(ek_pool* p, long long n)
@@ -311,27 +450,36 @@ extern "C" int ek_pool_create(const ek_params* kind, long long arena_bytes, int 
     g_pool_create_err = why;
     return EK_EINVAL;
   }
-  if (kind->mhd || kind->riemann != ek::RS_HLLC || !(kind->slope_type == 1.0 || kind->slope_type == 2.0)) {
-    g_pool_create_err = "the job pool runs hydro HLLC jobs with slope_type 1 or 2";
+  const bool mhd3 = kind->mhd && kind->dim == 3;
+  if ((kind->mhd && !mhd3) || (!kind->mhd && kind->riemann != ek::RS_HLLC) ||
+      !(kind->slope_type == 1.0 || kind->slope_type == 2.0)) {
+    g_pool_create_err = "the job pool runs hydro HLLC jobs (2D or 3D) and 3D MHD jobs, with slope_type 1 or 2";
     return EK_EINVAL;
   }
   ek_pool* p = new ek_pool();
   p->kind = *kind;
+  p->mhd3 = mhd3;
   p->fp64 = kind->precision == EK_FP64;
   p->elem = p->fp64 ? 8 : 4;
-  p->cap = arena_bytes / 2 / (long long)p->elem / kAlign * kAlign;
+  const double state_share = mhd3 ? kMhdStateShare : 1.0;
+  p->cap = (long long)(arena_bytes * state_share) / 2 / (long long)p->elem / kAlign * kAlign;
+  if (mhd3) {
+    p->wcap = (long long)(arena_bytes * (1.0 - state_share)) / (long long)p->elem / kAlign * kAlign;
+    p->wfree[0] = p->wcap;
+  }
   p->max_jobs = max_jobs;
   p->jobs.resize(max_jobs);
   p->freel[0] = p->cap;
-  p->variant = kind->dim == 2 ? 0 : 2; // 16x16 tiles (2D), 32-plane chunks (3D) until tuned
+  p->variant = (kind->dim == 2 || mhd3) ? 0 : 2; // 16x16 tiles (2D), 32-plane chunks (3D) until tuned
   int dev = 0;
   hipError_t e;
-  const size_t recb = p->fp64 ? sizeof(ek::PoolJob<double>) : sizeof(ek::PoolJob<float>);
+  const size_t recb = record_bytes(p);
   if ((e = hipGetDevice(&dev)) != hipSuccess ||
       (e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipMalloc(&p->A[0], (size_t)p->cap * p->elem)) != hipSuccess ||
       (e = hipMalloc(&p->A[1], (size_t)p->cap * p->elem)) != hipSuccess ||
+      (mhd3 && (e = hipMalloc(&p->W, (size_t)p->wcap * p->elem)) != hipSuccess) ||
       (e = hipMalloc(&p->ts, (size_t)max_jobs * 3 * sizeof(ek::TimeSlot))) != hipSuccess ||
       (e = hipMalloc(&p->djobs, (size_t)max_jobs * recb)) != hipSuccess ||
       (e = hipMalloc(&p->partial, (size_t)1024 * 16 * sizeof(double))) != hipSuccess ||
@@ -364,6 +512,8 @@ extern "C" void ek_pool_destroy(ek_pool* p)
   hipFree(p->ts);
   hipFree(p->djobs);
   hipFree(p->dwork);
+  hipFree(p->dworkU);
+  hipFree(p->W);
   hipFree(p->stage);
   hipFree(p->partial);
   if (p->stream)
@@ -385,7 +535,10 @@ extern "C" long long ek_pool_job_bytes(const ek_pool* p, const ek_params* j)
 {
   if (!p || !j)
     return -1;
-  return 2 * round_up(job_cells(*j) * j->nbvar) * (long long)p->elem;
+  long long n = 2 * round_up(job_cells(*j) * j->nbvar);
+  if (p->mhd3)
+    n += round_up(work_elems(*j));
+  return n * (long long)p->elem;
 }
 
 extern "C" long long ek_pool_free_bytes(const ek_pool* p)
@@ -395,7 +548,10 @@ extern "C" long long ek_pool_free_bytes(const ek_pool* p)
   long long f = 0;
   for (const auto& kv : p->freel)
     f += kv.second;
-  return 2 * f * (long long)p->elem;
+  f *= 2;
+  for (const auto& kv : p->wfree)
+    f += kv.second;
+  return f * (long long)p->elem;
 }
 
 extern "C" int ek_pool_active(const ek_pool* p)
@@ -457,12 +613,18 @@ static int admit_upload(ek_pool* p, const ek_params* jp, const double* host, int
     ek::Geom<double> g;
     ek::Gas<double> gs;
     ek::fill_geom<double>(*jp, g, gs);
-    PHIP(p, ek::fp64::inv_dt(jp->dim, (const double*)dst, g, gs, bits, nullptr, p->stream));
+    if (p->mhd3)
+      PHIP(p, ek::fp64::mhd3_inv_dt((const double*)dst, g, gs, bits, nullptr, p->stream));
+    else
+      PHIP(p, ek::fp64::inv_dt(jp->dim, (const double*)dst, g, gs, bits, nullptr, p->stream));
   } else {
     ek::Geom<float> g;
     ek::Gas<float> gs;
     ek::fill_geom<float>(*jp, g, gs);
-    PHIP(p, ek::fp32::inv_dt(jp->dim, (const float*)dst, g, gs, bits, nullptr, p->stream));
+    if (p->mhd3)
+      PHIP(p, ek::fp32::mhd3_inv_dt((const float*)dst, g, gs, bits, nullptr, p->stream));
+    else
+      PHIP(p, ek::fp32::inv_dt(jp->dim, (const float*)dst, g, gs, bits, nullptr, p->stream));
   }
   PHIP(p, hipStreamSynchronize(p->stream));
   if (p->test_fail_admit == 2)
@@ -492,9 +654,17 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
   if (id < 0)
     return pfail(p, EK_EFULL, "no free job slot");
   const long long ncell = job_cells(*jp), nelem = ncell * jp->nbvar, need = round_up(nelem);
+  const long long wneed = p->mhd3 ? round_up(work_elems(*jp)) : 0;
+  if (p->mhd3 && (double)(jp->nx + 6) * (jp->ny + 6) * (jp->nz + 6) * (double)p->elem >= 4294967296.0)
+    return pfail(p, EK_EINVAL, "3D MHD job too large: each padded work array must stay below 4 GiB");
   const long long off = arena_alloc(p, need);
   if (off < 0)
     return pfail(p, EK_EFULL, "arena full");
+  const long long woff = p->mhd3 ? list_alloc(p->wfree, wneed) : 0;
+  if (woff < 0) {
+    arena_free(p, off, need);
+    return pfail(p, EK_EFULL, "work buffer full");
+  }
   // The slot is filled but stays unused (out of the work list, status and downloads) until
   // every step has succeeded; any failure gives the arena range and the slot back, so the
   // pool is as it was before the call.
@@ -504,6 +674,8 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
   h.off = off;
   h.nelem = nelem;
   h.alloc = need;
+  h.woff = woff;
+  h.walloc = wneed;
   h.ncell = ncell;
   h.adm_step = p->nstep;
   h.nStepmax = nStepmax;
@@ -511,6 +683,8 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
   auto rollback = [&](int rc) {
     p->jobs[id] = HostJob{};
     arena_free(p, off, need);
+    if (wneed > 0)
+      list_free(p->wfree, woff, wneed);
     return rc;
   };
   if (int rc = admit_upload(p, jp, host, layout, id, off, nelem))
@@ -668,13 +842,15 @@ extern "C" int ek_pool_retire(ek_pool* p, int id)
     return pfail(p, EK_EINVAL, "invalid job");
   HostJob& h = p->jobs[id];
   arena_free(p, h.off, h.alloc);
+  if (h.walloc > 0)
+    list_free(p->wfree, h.woff, h.walloc);
   h = HostJob{};
   return rebuild_work(p);
 }
 
 static int apply_variant(ek_pool* p, int v)
 {
-  const int nvar = p->kind.dim == 2 ? ek::pool::N2VARIANTS : 4;
+  const int nvar = p->mhd3 ? 1 : (p->kind.dim == 2 ? ek::pool::N2VARIANTS : 4);
   if (v < 0 || v >= nvar)
     return pfail(p, EK_EINVAL, "tile choice out of range");
   p->variant = v;
@@ -697,7 +873,7 @@ extern "C" int ek_pool_autotune(ek_pool* p, int steps, int* choice, double* ms_o
 {
   if (!p || steps <= 0)
     return pfail(p, EK_EINVAL, "invalid argument");
-  const int nvar = p->kind.dim == 2 ? ek::pool::N2VARIANTS : 4;
+  const int nvar = p->mhd3 ? 1 : (p->kind.dim == 2 ? ek::pool::N2VARIANTS : 4);
   // snapshot of the current buffer and the time slots, restored after every trial
   long long hi = 0;
   for (const HostJob& h : p->jobs)

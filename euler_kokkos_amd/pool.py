@@ -1,9 +1,10 @@
 """Job pool (include/ek_pool.h) from Python, and the batched-run generator.
 
-A :class:`Pool` keeps many independent hydro grids resident in one device arena and
-advances all unfinished jobs by one step of their own dt per launch (a persistent kernel
-over a (job, tile) work list, replayed from a captured HIP graph). A job's result equals
-its single-grid :class:`native.Context` run.
+A :class:`Pool` keeps many independent grids (hydro 2D/3D, or 3D MHD) resident in one
+device arena and advances all unfinished jobs by one step of their own dt per pool step (a
+persistent kernel over a (job, tile) work list for hydro, the five 3D MHD pipeline kernels
+each over all jobs' chunks for MHD, replayed from a captured HIP graph). A job's result
+equals its single-grid :class:`native.Context` run.
 
 :func:`random_blast_batch` draws the batched workload (256 independent 2D blasts) from
 a seeded std::mt19937_64 stream with a hand-written uniform mapping, so the same seed

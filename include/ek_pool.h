@@ -3,7 +3,7 @@
  *
  * No counterpart in the reference, which runs one grid per process; the pool serves the
  * batched and streaming uses of the same MUSCL-Hancock step. Each job is what one
- * reference run of a Hydro_Muscl_2D / Hydro_Muscl_3D .ini would compute: its own grid,
+ * reference run of a Hydro_Muscl_2D / Hydro_Muscl_3D / MHD_Muscl_3D .ini would compute: its own grid,
  * boundaries, gas constants, CFL, nStepmax and tEnd, and its own device-resident time
  * loop (SolverBase::compute_dt / finished / next_iteration, src/shared/SolverBase.cpp:124-195).
  * A job's result is the one its single-grid ek_run gives (same per-tile code).
@@ -12,7 +12,10 @@
  * state in both, retirement frees it. One ek_pool_step launch advances every unfinished
  * job by one step of its own dt, walking a work list of (job, tile) items with a
  * persistent grid; consecutive steps over an unchanged job set are replayed from a
- * captured HIP graph. Tile shapes are chosen by ek_pool_autotune.
+ * captured HIP graph. Tile shapes are chosen by ek_pool_autotune. A 3D MHD pool also keeps
+ * each job's ghost-padded work area (78 reals per padded cell) in a work buffer that takes
+ * 84 % of arena_bytes (the two state buffers take the rest), and one pool step is the five
+ * kernels of the 3D MHD step, each a persistent grid over all active jobs.
  *
  * All calls return 0 or an EK_E* code (include/ek_hip.h) with a message from
  * ek_pool_last_error. One pool per host thread.
@@ -32,7 +35,8 @@ typedef struct ek_pool ek_pool;
 
 /* kind: the family every job of the pool belongs to -- dim, mhd, riemann, slope_type and
  * precision are taken from it (grid sizes, boundaries and constants may differ per job).
- * Supported: hydro 2D and 3D with HLLC, slope_type 1 or 2, FP64 or FP32.
+ * Supported: hydro 2D and 3D with HLLC, and 3D MHD with HLLD, HLL or LLF; slope_type 1 or 2;
+ * FP64 or FP32.
  * arena_bytes: device bytes for job states (split over the two ping-pong buffers). */
 int ek_pool_create(const ek_params* kind, long long arena_bytes, int max_jobs, ek_pool** out);
 void ek_pool_destroy(ek_pool* pool);

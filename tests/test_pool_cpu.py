@@ -42,3 +42,15 @@ def test_pool_symbols_exported():
     L = native.lib()
     for s in pool.POOL_EXPORTS:
         assert hasattr(L, s), s
+
+
+def test_pool_rejects_kinds_it_does_not_run():
+    # checked on the host before any device allocation
+    import pytest
+    from euler_kokkos_amd.native import EkError
+    from euler_kokkos_amd.params import make_params
+    for kind in (make_params("MHD_Muscl_2D", 32, 32, gamma0="1.6666667", riemann="hlld"),
+                 make_params("Hydro_Muscl_3D", 16, 16, 16, riemann="llf"),
+                 make_params("MHD_Muscl_3D", 16, 16, 16, gamma0="1.6666667", riemann="hlld", slope_type="0")):
+        with pytest.raises(EkError, match="job pool runs|slope_type"):
+            pool.Pool(kind, 1 << 20, 4)

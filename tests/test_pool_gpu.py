@@ -165,3 +165,42 @@ def test_failed_admission_leaves_the_pool_as_it_was(stage):
             assert np.array_equal(P.download(jid), ref[k][0]), k
     finally:
         P.close()
+
+
+def jobs_mhd3d(precision):
+    out = []
+    for nx, ny, nz, bc, B0 in [(24, 20, 16, (3,) * 6, 0.5), (16, 28, 20, (1, 1, 3, 3, 2, 2), 1.0),
+                               (32, 32, 32, (3,) * 6, 0.2)]:
+        p = make_params("MHD_Muscl_3D", nx, ny, nz, gamma0="1.6666667", cfl="0.8", slope_type="2",
+                        riemann="hlld", bc=bc, precision=precision)
+        out.append((p, problems.mhd_blast3d(p, B0=B0, radius="0.2", center=("0.45", "0.5", "0.55"))))
+    return out
+
+
+@pytest.mark.parametrize("precision", [native.EK_FP64, native.EK_FP32])
+def test_pool_mhd3d_matches_single_grid(precision):
+    # 3D MHD jobs of different shapes, boundaries and fields, admitted while others step and
+    # stopping at different nStepmax: each ends bitwise at its single-grid ek_run result
+    jobs = jobs_mhd3d(precision)
+    steps = [9, 14, 7]
+    ref = [single_run(p, U, s) for (p, U), s in zip(jobs, steps)]
+    P = pool.Pool(jobs[0][0], 512 << 20, 8)
+    try:
+        free0 = P.free_bytes()
+        ids = [P.admit(jobs[0][0], jobs[0][1], steps[0]), P.admit(jobs[1][0], jobs[1][1], steps[1])]
+        assert P.free_bytes() == free0 - P.job_bytes(jobs[0][0]) - P.job_bytes(jobs[1][0])
+        P.step(4)
+        ids.append(P.admit(jobs[2][0], jobs[2][1], steps[2]))
+        P.step(20)
+        for k, jid in enumerate(ids):
+            t, it, fin = P.status(jid)
+            G = P.download(jid)
+            print(f"case=pool mhd3d prec={precision} job={k}: it={it} max|diff|={np.abs(G - ref[k][0]).max():.3e}",
+                  flush=True)
+            assert fin and it == steps[k] == ref[k][2] and t == ref[k][1]
+            assert np.array_equal(G, ref[k][0])
+        for jid in ids:
+            P.retire(jid)
+        assert P.free_bytes() == free0 and P.active() == 0
+    finally:
+        P.close()
