@@ -84,6 +84,7 @@ def cpu_baseline(n, seconds):
     op = pyoracle.from_ek(ekp)
     U = pyoracle.blast(op, radius=0.1, center_x=0.5, center_y=0.5, center_z=0.5, density_in=1.0,
                        density_out=1.0, pressure_in=10.0, pressure_out=0.1)
+    host = host_info()  # before the first parallel region binds this thread to one place
     t0 = time.perf_counter()
     pyoracle.run(op, U, 1, 1e30)
     one = time.perf_counter() - t0
@@ -97,11 +98,18 @@ def cpu_baseline(n, seconds):
             "sample": f"oracle (C restatement of the reference impl-0 OpenMP step, gcc -O3 "
                       f"-ffp-contract=off) on the same {n}^3 blast, {steps} steps after 1 untimed step, "
                       f"{cores} OpenMP threads",
-            "ref_style_mcell_per_s": steps * (n + 4) ** 3 / el * 1e-6, **host_info()}
+            "ref_style_mcell_per_s": steps * (n + 4) ** 3 / el * 1e-6, **host}
+
+
+_HOST = None
 
 
 def host_info():
-    """CPU model and OpenMP placement of the host the CPU baseline runs on."""
+    """CPU model and OpenMP placement of the host the CPU baseline runs on (taken once, at the
+    first call, before any OpenMP region binds the calling thread to one place)."""
+    global _HOST
+    if _HOST is not None:
+        return dict(_HOST)
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -110,9 +118,10 @@ def host_info():
                 break
     except OSError:
         pass
-    return {"cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
-            "host_cpus_available": len(os.sched_getaffinity(0))}
+    _HOST = {"cpu_model": model, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+             "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS"),
+             "host_cpus_available": len(os.sched_getaffinity(0)), "host_cpus_online": os.cpu_count()}
+    return dict(_HOST)
 
 
 def mhd3d_c5(n, steps, with_cpu, cpu_seconds):
@@ -312,6 +321,11 @@ def batch_c4(n_runs, steps, seed):
     return res
 
 
+def progress(msg):
+    """One line per finished stage on stderr (stdout carries only the JSON result line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def serving_c5(n_jobs, rate, seed, steps):
     """Serving stream: seeded Poisson arrivals of 3D jobs (edges uniform in [64,192], `steps`
     steps each), admitted into device-resident FP32 job pools by HBM footprint and advanced
@@ -325,6 +339,7 @@ def serving_c5(n_jobs, rate, seed, steps):
                        f"FP32 state, FP64 dt",
            "kinds": ["mhd_blast", "hydro_kh"]}
     jobs = serving.poisson_stream(n_jobs, rate, seed, steps=steps, precision=native.EK_FP32)
+    progress(f"serving: {n_jobs} job states built")
     for name, burst in (("stream", False), ("burst", True)):
         # stream: the Poisson arrival times (latency at the offered load); burst: every job
         # arrives at t = 0 (throughput at saturation)
@@ -332,14 +347,16 @@ def serving_c5(n_jobs, rate, seed, steps):
         if burst:
             for j in jobs:
                 j.arrival = 0.0
-        S = serving.Server(48 << 30, native.EK_FP32)
+        S = serving.Server(128 << 30, native.EK_FP32)
         try:
             fp32 = S.serve(jobs)
         finally:
             S.close()
+        progress(f"serving {name}: FP32 continuous batching done ({fp32.get('jobs_per_s', 0):.1f} jobs/s)")
         for j in jobs:
             j.done_t = -1.0
         fp64 = serving.serve_one_at_a_time(jobs, native.EK_FP64)
+        progress(f"serving {name}: FP64 one at a time done ({fp64.get('jobs_per_s', 0):.1f} jobs/s)")
         for j, a in zip(jobs, arr):
             j.arrival = a
             j.done_t = -1.0
@@ -413,7 +430,7 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--batch-seed", type=int, default=2024)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-stream line")
-    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-jobs", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seed", type=int, default=77)
     a = ap.parse_args()
@@ -426,6 +443,7 @@ def main() -> int:
         copy_calibration()
         return 0
 
+    host_info()
     from euler_kokkos_amd import native, problems
     from euler_kokkos_amd.params import make_params
     dev = native.device_info()
@@ -435,6 +453,7 @@ def main() -> int:
     U0 = problems.blast(ekp64, radius="0.1", density_in="1", density_out="1", pressure_in="10",
                         pressure_out="0.1")
     r64 = gpu_run(ekp64, U0, a.warmup, a.steps)
+    progress(f"headline FP64 {n}^3: {r64['wall'] / a.steps * 1e3:.4f} ms/step")
     cells = n ** 3
     value = a.steps * cells / r64["wall"]
     avg_ms = r64["ev_ms"] / max(r64["launches"], 1)
@@ -483,16 +502,21 @@ def main() -> int:
             "l1_rho": float(np.abs(d32[0] - d64[0]).mean()),
             "l1_E_rel": float(np.abs(d32[1] - d64[1]).mean() / np.abs(d64[1]).mean()),
         }
+        progress("headline FP32 done")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, a.cpu_seconds)
+        progress("CPU baseline done")
     else:
         out["cpu_baseline"] = None
     if not a.no_mhd:
         out["mhd2d"] = mhd_c3(a.mhd_steps, not a.no_cpu, a.cpu_seconds / 2)
+        progress("2D MHD done")
     if not a.no_mhd3d:
         out["mhd3d"] = mhd3d_c5(a.mhd3d_n, a.mhd3d_steps, not a.no_cpu, a.cpu_seconds / 2)
+        progress("3D MHD done")
     if not a.no_batch:
         out["batch2d"] = batch_c4(a.batch_runs, a.batch_steps, a.batch_seed)
+        progress("batched 2D done")
     if not a.no_serve:
         out["serving3d"] = serving_c5(a.serve_jobs, a.serve_rate, a.serve_seed, 50)
     print(json.dumps(out))

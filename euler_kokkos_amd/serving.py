@@ -4,20 +4,20 @@ their HBM footprint and advances every admitted job together (continuous batchin
 Jobs arrive on a seeded Poisson process. Each job kind has its own pool and its own loop,
 the loops running side by side: at every turn a loop admits, in arrival order, each job of
 its kind that has arrived and fits in the pool's memory; then it advances all admitted jobs
-of the pool together (one launch for a hydro pool, every context's stream at once for 3D
-MHD, whose contexts are created and filled on a background thread); finished jobs are
-collected and retired, which frees memory for the queue. Time-to-solution is measured from
-a job's arrival to its completion; throughput over the whole stream.
+of the pool together (one persistent launch per step for a hydro pool, the five 3D MHD
+kernels each over all jobs for an MHD pool, replayed from a captured graph); finished jobs
+are collected and retired, which frees memory for the queue. Time-to-solution is measured
+from a job's arrival to its completion; throughput over the whole stream.
 
 A job's result does not depend on what else runs with it: the pool steps each job with
-the per-tile code of its single-grid run (see tests/test_serving_gpu.py).
+the per-cell code of its single-grid run (see tests/test_serving_gpu.py).
 """
 from __future__ import annotations
 
 import math
 import time
 from collections import deque
-from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -109,121 +109,6 @@ def poisson_stream(n_jobs: int, rate: float, seed: int, kinds=("hydro_kh", "mhd_
     return jobs
 
 
-class ContextGroup:
-    """Jobs that each run in their own device context (3D MHD: the step pipeline needs a
-    per-job ghost-padded work area), admitted against a byte budget. Same interface as a
-    Pool: admit / step / status / download / retire. Admission reserves the job's bytes at
-    once and creates and uploads its context on a background thread, so the host-to-device
-    copy of a new job overlaps the stepping of the admitted ones; a job steps once its
-    context is ready. A step advances every ready, unfinished job on its own context's
-    stream from a small thread pool (the C calls release the GIL), so small grids overlap on
-    the GPU instead of running one after another."""
-
-    WORK_ARRAYS = 78  # reals per padded cell in the 3D MHD work area (mhd3::A_TOTAL)
-
-    def __init__(self, budget_bytes: int, workers: int = 8):
-        self.budget = int(budget_bytes)
-        self.used = 0
-        self.jobs = {}
-        self.next_id = 0
-        self._ex = ThreadPoolExecutor(max_workers=workers)
-        self._up = ThreadPoolExecutor(max_workers=2)
-
-    def job_bytes(self, p) -> int:
-        elem = 8 if p.precision == native.EK_FP64 else 4
-        cells = p.nx * p.ny * p.nz
-        padded = (p.nx + 6) * (p.ny + 6) * (p.nz + 6)
-        return elem * (2 * 8 * cells + self.WORK_ARRAYS * padded) + (1 << 20)
-
-    @staticmethod
-    def _create(p, U):
-        ctx = native.Context(p)
-        try:
-            ctx.upload(U, native.EK_LAYOUT_SOA)
-            ctx.sync()
-        except Exception:
-            ctx.close()
-            raise
-        return ctx
-
-    def admit(self, p, U, nStepmax, tEnd=1e30):
-        need = self.job_bytes(p)
-        if self.used + need > self.budget:
-            raise PoolFull("context group budget exhausted")
-        jid = self.next_id
-        self.next_id += 1
-        self.jobs[jid] = dict(ctx=None, fut=self._up.submit(self._create, p, U), t=0.0, it=0, nStepmax=nStepmax,
-                              tEnd=tEnd, need=need, error=None)
-        self.used += need
-        return jid
-
-    @staticmethod
-    def _ready(j) -> bool:
-        # a context that could not be created fails its job alone (failed() reports why)
-        if j["ctx"] is None and j["error"] is None and j["fut"].done():
-            exc = j["fut"].exception()
-            if exc is None:
-                j["ctx"] = j["fut"].result()
-            else:
-                j["error"] = exc
-        return j["ctx"] is not None
-
-    def step(self, nsteps=1):
-        live = [j for j in self.jobs.values()
-                if j["error"] is None and j["it"] < j["nStepmax"] and j["t"] < j["tEnd"] - 1e-14]
-        todo = [j for j in live if self._ready(j)]
-        if not todo and live:
-            # nothing uploaded yet: wait for the first context instead of spinning
-            wait([j["fut"] for j in live], return_when=FIRST_COMPLETED)
-            todo = [j for j in live if self._ready(j)]
-
-        def one(j):
-            # a job whose step fails (a HIP error on its context) fails alone: status() then
-            # reports it done and failed() returns the error
-            try:
-                j["t"], j["it"], _ = j["ctx"].run(nsteps, j["nStepmax"], j["tEnd"], j["t"], j["it"])
-            except Exception as e:  # noqa: BLE001 -- any failure of this job's context
-                j["error"] = e
-
-        if len(todo) == 1:
-            one(todo[0])
-        elif todo:
-            list(self._ex.map(one, todo))
-
-    def status(self, jid):
-        j = self.jobs[jid]
-        self._ready(j)
-        done = j["error"] is not None or j["it"] >= j["nStepmax"] or j["t"] >= j["tEnd"] - 1e-14
-        return j["t"], j["it"], done
-
-    def failed(self, jid):
-        """The exception that failed the job, or None."""
-        j = self.jobs[jid]
-        self._ready(j)
-        return j["error"]
-
-    def download(self, jid):
-        j = self.jobs[jid]
-        if j["ctx"] is None:
-            j["ctx"] = j["fut"].result()  # raises the job's error
-        return j["ctx"].download(native.EK_LAYOUT_SOA)
-
-    def retire(self, jid):
-        j = self.jobs.pop(jid)
-        self.used -= j["need"]
-        try:
-            ctx = j["ctx"] if j["ctx"] is not None else j["fut"].result()
-        except Exception:
-            return  # the context was never made
-        ctx.close()
-
-    def close(self):
-        for jid in list(self.jobs):
-            self.retire(jid)
-        self._ex.shutdown()
-        self._up.shutdown()
-
-
 class Server:
     """Job pools per kind sharing an HBM budget; continuous batching over arrivals. Each
     kind's pool has its own loop (admission in arrival order as memory allows, then all of
@@ -234,10 +119,7 @@ class Server:
         self.pools = {}
         share = hbm_bytes // len(kinds)
         for k in kinds:
-            if k == "mhd_blast":
-                self.pools[k] = ContextGroup(share)
-            else:
-                self.pools[k] = Pool(job_params(k, 64, precision), share, max_jobs)
+            self.pools[k] = Pool(job_params(k, 64, precision), share, max_jobs)
         self.precision = precision
         self.max_chunk = max_chunk  # most steps advanced per loop turn (between admissions)
         self._ex = ThreadPoolExecutor(max_workers=len(self.pools))

@@ -96,65 +96,3 @@ def test_serving_loop_fails_bad_jobs_alone():
         else:
             assert j.error and j.result is None
     assert not S.pools["hydro_kh"].jobs and not S.pools["mhd_blast"].jobs
-
-
-def test_context_group_job_that_cannot_be_created_fails_alone():
-    # ek_create refuses the 3D MHD trace's slope_type 0 before touching the device: the
-    # job is reported failed, the group keeps serving and its memory share comes back
-    from euler_kokkos_amd.params import make_params
-    G = serving.ContextGroup(1 << 40)
-    try:
-        p = make_params("MHD_Muscl_3D", 8, 8, 8, slope_type="0", riemann="hlld", gamma0="1.6666667")
-        U = np.zeros((8, 8, 8, 8))
-        jid = G.admit(p, U, 3)
-        assert G.used > 0
-        G.step(1)  # waits for the creation attempt
-        _, it, fin = G.status(jid)
-        assert fin and it == 0
-        err = G.failed(jid)
-        assert isinstance(err, native.EkError) and "slope_type" in str(err)
-        G.retire(jid)
-        assert G.used == 0 and not G.jobs
-    finally:
-        G.close()
-
-
-class _FakeCtx:
-    """A stepping context: run() advances the iteration, or raises for a failing job."""
-
-    def __init__(self, fail):
-        self.fail = fail
-        self.closed = False
-
-    def run(self, n, nStepmax, tEnd, t, it):
-        if self.fail:
-            raise native.EkError("ek_run failed (2): injected HIP error")
-        it2 = min(nStepmax, it + n)
-        return t + 0.1 * (it2 - it), it2, 0.1
-
-    def download(self, layout=None):
-        return np.zeros(3)
-
-    def close(self):
-        self.closed = True
-
-
-def test_context_group_step_failure_fails_that_job_alone():
-    # one context's run raises during a step: that job alone is reported done and failed,
-    # the other jobs keep stepping to completion in the same and later steps
-    G = serving.ContextGroup(1 << 40)
-    G._create = staticmethod(lambda p, U: _FakeCtx(fail=(p == "bad")))
-    try:
-        G.job_bytes = lambda p: 1
-        ids = [G.admit(p, None, 4) for p in ("ok", "bad", "ok")]
-        for _ in range(6):
-            G.step(1)
-        for jid, p in zip(ids, ("ok", "bad", "ok")):
-            _, it, fin = G.status(jid)
-            assert fin
-            if p == "bad":
-                assert isinstance(G.failed(jid), native.EkError) and it == 0
-            else:
-                assert G.failed(jid) is None and it == 4
-    finally:
-        G.close()

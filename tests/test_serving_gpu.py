@@ -15,7 +15,7 @@ def small_stream(seed=5, edge=(20, 40)):
 
 
 def alone(j):
-    P = Pool(j.params, 64 << 20, 4) if j.kind == "hydro_kh" else serving.ContextGroup(1 << 30)
+    P = Pool(j.params, (64 if j.kind == "hydro_kh" else 512) << 20, 4)
     try:
         jid = P.admit(j.params, j.U0, j.steps)
         P.step(j.steps + 1)
@@ -60,9 +60,9 @@ def test_admission_waits_for_memory():
 
 
 def test_bad_jobs_fail_alone_in_the_mix():
-    # one hydro job the pool refuses at admission and one MHD job whose context cannot be
-    # created (slope_type 0 is not an MHD trace option): both end with an error, every other
-    # job's result is still bitwise its result alone
+    # one hydro and one MHD job that their pools refuse at admission (slope_type 0 is not
+    # the pools' kind, and not an MHD trace option at all): both end with an error, every
+    # other job's result is still bitwise its result alone
     jobs = small_stream(seed=11)
     bad = {next(j.id for j in jobs if j.kind == "hydro_kh"), next(j.id for j in jobs if j.kind == "mhd_blast")}
     for j in jobs:
