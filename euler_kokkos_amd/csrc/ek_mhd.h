@@ -492,8 +492,10 @@ __device__ __forceinline__ void floor_dp(const Gas<T>& g, T& r, T& p)
   p = emax(g.smallp * r, p);
 }
 
-// At most two waves per SIMD (two blocks per CU): the FP64 kernel is there already by its
-// shared memory; the FP32 one measures 20% faster at two than at the four its registers allow.
+// Registers for two waves per SIMD. FP64 14x14 (256 threads, 81 KB of LDS) runs two blocks
+// per CU, two waves per SIMD, set by its LDS. With the FP32 14x14 tile, two waves per SIMD
+// measured 20% faster than the four its registers allow; the FP32 step now uses the 30x14
+// tile (512 threads, 82 KB), two blocks per CU.
 template <class T, int TX, int TY, int NT, int RS, int ST>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(1, 2))) void mhd2d_step(const T* __restrict__ Uin, T* __restrict__ Uout, Geom<T> geo,
                                                  Gas<T> gas, StepCtl ctl)
