@@ -236,34 +236,32 @@ __global__ __launch_bounds__(256) void efield_kernel(T* __restrict__ S, Work3 w,
   efield_cell<T, ST>(S, w, i, j, k);
 }
 
-// the 3D trace (trace_unsplit_mhd_3d_simpler) on 1 <= i,j,k < size-2
-template <class T, int ST>
-__device__ __forceinline__ void trace_cell(T* __restrict__ S, const Work3& w, const Geom<T>& geo, const Gas<T>& gas,
-                                           double dt, int i, int j, int k)
+// Inputs of the 3D trace of one cell, gathered from wherever the caller keeps them.
+template <class T>
+struct TraceIn {
+  T qc[NV];                  // primitives of the cell
+  T dqx[NV], dqy[NV], dqz[NV];  // their slopes (slope1 of the neighbours along x, y, z)
+  T ELL, ELR, ERL, ERR;      // Ex at (j, k), (j, k+1), (j+1, k), (j+1, k+1)
+  T FLL, FLR, FRL, FRR;      // Ey at (i, k), (i, k+1), (i+1, k), (i+1, k+1)
+  T GLL, GLR, GRL, GRR;      // Ez at (i, j), (i, j+1), (i+1, j), (i+1, j+1)
+  T AL, AR, BL, BR, CL, CR;  // the cell's own face fields (left/right)
+};
+
+// The half-step record of one cell (trace_unsplit_mhd_3d_simpler): `put(k, value)` receives
+// the 35 values K_C .. K_Z+6.
+template <class T, int ST, class Put>
+__device__ __forceinline__ void trace_record(const TraceIn<T>& in, const Gas<T>& gas, T dtdx, T dtdy, T dtdz,
+                                             Put put)
 {
-  if (i < 1 || j < 1 || k < 1 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
-    return;
-  const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
-  const long long PN = w.PN, o = pidx(w, i, j, k);
-  const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
-  const T* Q = S + A_Q * PN;
-  const T* Up = S + A_UP * PN;
-  const T* E = S + A_E * PN;
-  const T* DB = S + A_DB * PN;
-  T qc[NV], dqx[NV], dqy[NV], dqz[NV];
-#pragma unroll
-  for (int v = 0; v < NV; v++) {
-    qc[v] = Q[v * PN + o];
-    dqx[v] = slope1<T, ST>(qc[v], Q[v * PN + o + sx], Q[v * PN + o - sx]);
-    dqy[v] = slope1<T, ST>(qc[v], Q[v * PN + o + sy], Q[v * PN + o - sy]);
-    dqz[v] = slope1<T, ST>(qc[v], Q[v * PN + o + sz], Q[v * PN + o - sz]);
-  }
-  const T ELL = E[o], ELR = E[o + sz], ERL = E[o + sy], ERR = E[o + sy + sz];
-  const T FLL = E[PN + o], FLR = E[PN + o + sz], FRL = E[PN + o + sx], FRR = E[PN + o + sx + sz];
-  const T GLL = E[2 * PN + o], GLR = E[2 * PN + o + sy], GRL = E[2 * PN + o + sx], GRR = E[2 * PN + o + sx + sy];
+  const T* qc = in.qc;
+  const T* dqx = in.dqx;
+  const T* dqy = in.dqy;
+  const T* dqz = in.dqz;
+  const T ELL = in.ELL, ELR = in.ELR, ERL = in.ERL, ERR = in.ERR;
+  const T FLL = in.FLL, FLR = in.FLR, FRL = in.FRL, FRR = in.FRR;
+  const T GLL = in.GLL, GLR = in.GLR, GRL = in.GRL, GRR = in.GRR;
   T r = qc[ID], p = qc[IP], u = qc[IU], v = qc[IV], ww = qc[IW], A = qc[IA], B = qc[IB], C = qc[IC];
-  T AL = Up[IA * PN + o], AR = Up[IA * PN + o + sx], BL = Up[IB * PN + o], BR = Up[IB * PN + o + sy];
-  T CL = Up[IC * PN + o], CR = Up[IC * PN + o + sz];
+  T AL = in.AL, AR = in.AR, BL = in.BL, BR = in.BR, CL = in.CL, CR = in.CR;
   const T h = T(0.5);
   const T drx = dqx[ID] * h, dpx = dqx[IP] * h, dux = dqx[IU] * h, dvx = dqx[IV] * h, dwx = dqx[IW] * h;
   const T dCx = dqx[IC] * h, dBx = dqx[IB] * h;
@@ -271,10 +269,6 @@ __device__ __forceinline__ void trace_cell(T* __restrict__ S, const Work3& w, co
   const T dCy = dqy[IC] * h, dAy = dqy[IA] * h;
   const T drz = dqz[ID] * h, dpz = dqz[IP] * h, duz = dqz[IU] * h, dvz = dqz[IV] * h, dwz = dqz[IW] * h;
   const T dAz = dqz[IA] * h, dBz = dqz[IB] * h;
-  const T dALy = h * DB[o], dALz = h * DB[PN + o], dBLx = h * DB[2 * PN + o], dBLz = h * DB[3 * PN + o];
-  const T dCLx = h * DB[4 * PN + o], dCLy = h * DB[5 * PN + o];
-  const T dARy = h * DB[o + sx], dARz = h * DB[PN + o + sx], dBRx = h * DB[2 * PN + o + sy];
-  const T dBRz = h * DB[3 * PN + o + sy], dCRx = h * DB[4 * PN + o + sz], dCRy = h * DB[5 * PN + o + sz];
   const T dAx = h * (AR - AL), dBy = h * (BR - BL), dCz = h * (CR - CL);
   const T gamma = gas.gamma0;
   const T sr0 = (-u * drx - dux * r) * dtdx + (-v * dry - dvy * r) * dtdy + (-ww * drz - dwz * r) * dtdz;
@@ -309,8 +303,6 @@ __device__ __forceinline__ void trace_cell(T* __restrict__ S, const Work3& w, co
   BR = BR + sBR0;
   CL = CL + sCL0;
   CR = CR + sCR0;
-  T* Rw = S + (long long)A_REC * PN + o;
-  auto put = [&](int kk, T x) { Rw[(long long)kk * PN] = x; };
   put(K_C + ID, r);
   put(K_C + IP, p);
   put(K_C + IU, u);
@@ -346,6 +338,50 @@ __device__ __forceinline__ void trace_cell(T* __restrict__ S, const Work3& w, co
   put(K_Z + 4, dwz);
   put(K_Z + 5, dAz);
   put(K_Z + 6, dBz);
+}
+
+// the 3D trace (trace_unsplit_mhd_3d_simpler) on 1 <= i,j,k < size-2
+template <class T, int ST>
+__device__ __forceinline__ void trace_cell(T* __restrict__ S, const Work3& w, const Geom<T>& geo, const Gas<T>& gas,
+                                           double dt, int i, int j, int k)
+{
+  if (i < 1 || j < 1 || k < 1 || i >= w.PX - 2 || j >= w.PY - 2 || k >= w.PZ - 2)
+    return;
+  const T dtdx = T(dt / geo.dx), dtdy = T(dt / geo.dy), dtdz = T(dt / geo.dz);
+  const long long PN = w.PN, o = pidx(w, i, j, k);
+  const long long sx = 1, sy = w.PX, sz = (long long)w.PX * w.PY;
+  const T* Q = S + A_Q * PN;
+  const T* Up = S + A_UP * PN;
+  const T* E = S + A_E * PN;
+  TraceIn<T> in;
+#pragma unroll
+  for (int v = 0; v < NV; v++) {
+    const T q = Q[v * PN + o];
+    in.qc[v] = q;
+    in.dqx[v] = slope1<T, ST>(q, Q[v * PN + o + sx], Q[v * PN + o - sx]);
+    in.dqy[v] = slope1<T, ST>(q, Q[v * PN + o + sy], Q[v * PN + o - sy]);
+    in.dqz[v] = slope1<T, ST>(q, Q[v * PN + o + sz], Q[v * PN + o - sz]);
+  }
+  in.ELL = E[o];
+  in.ELR = E[o + sz];
+  in.ERL = E[o + sy];
+  in.ERR = E[o + sy + sz];
+  in.FLL = E[PN + o];
+  in.FLR = E[PN + o + sz];
+  in.FRL = E[PN + o + sx];
+  in.FRR = E[PN + o + sx + sz];
+  in.GLL = E[2 * PN + o];
+  in.GLR = E[2 * PN + o + sy];
+  in.GRL = E[2 * PN + o + sx];
+  in.GRR = E[2 * PN + o + sx + sy];
+  in.AL = Up[IA * PN + o];
+  in.AR = Up[IA * PN + o + sx];
+  in.BL = Up[IB * PN + o];
+  in.BR = Up[IB * PN + o + sy];
+  in.CL = Up[IC * PN + o];
+  in.CR = Up[IC * PN + o + sz];
+  T* Rw = S + (long long)A_REC * PN + o;
+  trace_record<T, ST>(in, gas, dtdx, dtdy, dtdz, [&](int kk, T x) { Rw[(long long)kk * PN] = x; });
 }
 
 template <class T, int ST>
