@@ -9,12 +9,16 @@ namespace mhd {
 
 // Tile of the 2D MHD step: the (TX+2) x (TY+2) trace region gives one record per thread.
 // FP64: 14x14 cells, 256 threads (81 KB of LDS, two blocks per CU). FP32: 30x14, 512
-// threads (82 KB, two blocks per CU): less halo recomputation per cell, 0.097 against
-// 0.139 ms per 1024^2 Orszag-Tang step (in FP64 the large tile measured 0.260 against
-// 0.246 ms).
+// threads, compact LDS layout (53.7 KB, three blocks per CU, 71 VGPRs, six waves per SIMD).
+// Per 1024^2 Orszag-Tang step: FP32 0.139 ms at 14x14, 0.097 at 30x14, 0.088 with the
+// compact layout; FP64 0.246 ms at 14x14, 0.260 at 30x14, 0.249 with the compact layout at
+// three waves per SIMD (every variant bitwise the same).
 template <class T>
 struct M2Tile {
-  static constexpr int X = sizeof(T) == 8 ? 14 : 30, Y = 14, NT = sizeof(T) == 8 ? 256 : 512;
+  static constexpr bool F64 = sizeof(T) == 8;
+  static constexpr int X = F64 ? 14 : 30, Y = 14, NT = F64 ? 256 : 512;
+  static constexpr bool CMP = !F64;
+  static constexpr int WPE = F64 ? 2 : 6;
 };
 
 template <class T>
@@ -25,8 +29,8 @@ inline MStepFn<T> pick(int st)
 {
   using M = M2Tile<T>;
   if (st == 1)
-    return mhd2d_step<T, M::X, M::Y, M::NT, RS, 1>;
-  return mhd2d_step<T, M::X, M::Y, M::NT, RS, 2>;
+    return mhd2d_step<T, M::X, M::Y, M::NT, RS, 1, M::CMP, M::WPE>;
+  return mhd2d_step<T, M::X, M::Y, M::NT, RS, 2, M::CMP, M::WPE>;
 }
 
 template <class T>
