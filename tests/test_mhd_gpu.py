@@ -145,3 +145,29 @@ def test_orszag_tang_fp32_l1():
     print(f"case=OT 256^2 100 steps FP32 vs FP64: rel L1(rho)={l1:.3e}", flush=True)
     assert np.isfinite(d32).all()
     assert l1 <= 1e-4
+
+
+@pytest.mark.parametrize("bc", [(1, 1, 1, 1), (2, 2, 2, 2), (1, 2, 3, 3), (3, 3, 2, 1)])
+def test_mixed_boundaries_fp32_l1(bc):
+    # the FP32 step's 30x14 tiles on a grid that leaves partial tiles on both axes, with
+    # reflecting / outflow / periodic sides: relative L1 of every variable against the
+    # FP64 step (itself bitwise the oracle's on these boundaries) within 1e-4
+    ekp64 = mparams(97, 61, bc=bc + (3, 3))
+    ekp32 = mparams(97, 61, bc=bc + (3, 3), precision=native.EK_FP32)
+    U0 = problems.orszag_tang(ekp64)
+    out = []
+    for ekp in (ekp64, ekp32):
+        ctx = native.Context(ekp)
+        try:
+            ctx.upload(U0, native.EK_LAYOUT_SOA)
+            t, it, _ = ctx.run(40, 40, 1e30)
+            out.append(ctx.download(native.EK_LAYOUT_SOA))
+        finally:
+            ctx.close()
+        assert it == 40
+    d64, d32 = out
+    assert np.isfinite(d32).all()
+    for v in (ID, IP, IU, IV, IA, IB):
+        l1 = np.abs(d32[v] - d64[v]).mean() / np.abs(d64[v]).mean()
+        print(f"case=mhd2d FP32 bc={bc} var={v}: rel L1 {l1:.3e}", flush=True)
+        assert l1 <= 1e-4
