@@ -17,6 +17,12 @@ namespace {
 
 struct HostJob {
   bool used = false;
+  // admitted by ek_pool_admit_async and still uploading on the upload stream: not in the work
+  // lists until activate_pending sees its event; upar = the arena buffer it is uploaded into
+  bool pending = false;
+  int upar = 0;
+  long long seq = 0; // admission order
+  long long join_after = 0; // no earlier than this ek_pool_step call (ek_pool_test_defer_join)
   ek_params p{};
   long long off = 0, nelem = 0, alloc = 0;
   long long woff = 0, walloc = 0; // 3D MHD: the job's padded work area in the work buffer
@@ -81,6 +87,16 @@ struct ek_pool {
   int workU_cap = 0;
   int mgrid[5] = { 1, 1, 1, 1, 1 };
   int test_fail_admit = 0; // ek_pool_test_fail_admit: 0 off, else the admission step that fails
+  // asynchronous admission: uploads (and the FP32 rounding) run on their own stream, with
+  // their own staging buffer, one completion event per job slot
+  hipStream_t ustream = nullptr;
+  double* ustage = nullptr;
+  long long ustage_elems = 0;
+  std::vector<hipEvent_t> jev;
+  long long admissions = 0;
+  long long step_calls = 0;
+  int test_defer_join = 0; // ek_pool_test_defer_join
+  ek::TimeSlot zslots[3] = {}; // t = 0, it = 0: the source of every job's time-slot reset
 };
 
 static int pfail(ek_pool* p, int code, const std::string& m)
@@ -227,7 +243,7 @@ static int rebuild_work_mhd3(ek_pool* p)
   std::vector<ek::PoolWork> wp, wu;
   for (int id = 0; id < (int)p->jobs.size(); id++) {
     const HostJob& h = p->jobs[id];
-    if (!h.used || h.finished)
+    if (!h.used || h.finished || h.pending)
       continue;
     const long long pn = (long long)(h.p.nx + 6) * (h.p.ny + 6) * (long long)(h.p.nz + 6);
     const long long np = (pn + ek::MP_CHUNK - 1) / ek::MP_CHUNK, nu = (h.ncell + ek::MU_CHUNK - 1) / ek::MU_CHUNK;
@@ -253,7 +269,7 @@ static int rebuild_work(ek_pool* p)
   std::vector<ek::PoolWork> w;
   for (int id = 0; id < (int)p->jobs.size(); id++) {
     const HostJob& h = p->jobs[id];
-    if (!h.used || h.finished)
+    if (!h.used || h.finished || h.pending)
       continue;
     int ntx, nty, ntz;
     tiles_of(p, h.p, ntx, nty, ntz);
@@ -356,7 +372,7 @@ static int refresh_status(ek_pool* p, bool& changed)
     HostJob& j = p->jobs[id];
     // a finished job has left the work list: its slots are no longer forwarded, so its
     // t / it / finished stay as the refresh that saw it finish left them
-    if (!j.used || j.finished)
+    if (!j.used || j.finished || j.pending)
       continue;
     const ek::TimeSlot& s = h[(size_t)id * 3 + slot];
     j.t = s.t;
@@ -469,6 +485,7 @@ extern "C" int ek_pool_create(const ek_params* kind, long long arena_bytes, int 
   }
   p->max_jobs = max_jobs;
   p->jobs.resize(max_jobs);
+  p->jev.assign(max_jobs, nullptr);
   p->freel[0] = p->cap;
   p->variant = (kind->dim == 2 || mhd3) ? 0 : 2; // 16x16 tiles (2D), 32-plane chunks (3D) until tuned
   int dev = 0;
@@ -477,6 +494,7 @@ extern "C" int ek_pool_create(const ek_params* kind, long long arena_bytes, int 
   if ((e = hipGetDevice(&dev)) != hipSuccess ||
       (e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&p->ustream, hipStreamNonBlocking)) != hipSuccess ||
       (e = hipMalloc(&p->A[0], (size_t)p->cap * p->elem)) != hipSuccess ||
       (e = hipMalloc(&p->A[1], (size_t)p->cap * p->elem)) != hipSuccess ||
       (mhd3 && (e = hipMalloc(&p->W, (size_t)p->wcap * p->elem)) != hipSuccess) ||
@@ -499,8 +517,13 @@ extern "C" void ek_pool_destroy(ek_pool* p)
 {
   if (!p)
     return;
+  if (p->ustream)
+    hipStreamSynchronize(p->ustream);
   if (p->stream)
     hipStreamSynchronize(p->stream);
+  for (hipEvent_t ev : p->jev)
+    if (ev)
+      hipEventDestroy(ev);
   if (p->graph)
     hipGraphExecDestroy(p->graph);
   if (p->ev0)
@@ -515,9 +538,12 @@ extern "C" void ek_pool_destroy(ek_pool* p)
   hipFree(p->dworkU);
   hipFree(p->W);
   hipFree(p->stage);
+  hipFree(p->ustage);
   hipFree(p->partial);
   if (p->stream)
     hipStreamDestroy(p->stream);
+  if (p->ustream)
+    hipStreamDestroy(p->ustream);
   delete p;
 }
 
@@ -567,6 +593,8 @@ extern "C" int ek_pool_active(const ek_pool* p)
 // Uploads the admitted job's initial state into arena buffer `par` at its offset, resets its
 // time slots and reduces its first invDt. Touches nothing outside the job's own arena
 // range, slots and the staging buffer.
+static int init_slots(ek_pool* p, const ek_params* jp, int id, const char* dst);
+
 static int admit_upload(ek_pool* p, const ek_params* jp, const double* host, int layout, int id, long long off,
                         long long nelem)
 {
@@ -604,10 +632,19 @@ static int admit_upload(ek_pool* p, const ek_params* jp, const double* host, int
     return pfail(p, EK_EINVAL, "unknown layout");
   }
   PHIP(p, hipStreamSynchronize(p->stream));
-  // time slots: t = 0, it = 0 in the slot the next pool step reads, then its invDt
-  ek::TimeSlot init[3];
-  std::memset(init, 0, sizeof(init));
-  PHIP(p, hipMemcpyAsync(p->ts + 3 * id, init, sizeof(init), hipMemcpyHostToDevice, p->stream));
+  if (int rc = init_slots(p, jp, id, dst))
+    return rc;
+  PHIP(p, hipStreamSynchronize(p->stream));
+  if (p->test_fail_admit == 2)
+    return pfail(p, EK_EHIP, "injected admission failure (after the upload)");
+  return EK_OK;
+}
+
+// Time slots of job id on the pool stream: t = 0, it = 0, then the invDt of its state at dst
+// in the slot the next pool step reads. Does not synchronize.
+static int init_slots(ek_pool* p, const ek_params* jp, int id, const char* dst)
+{
+  PHIP(p, hipMemcpyAsync(p->ts + 3 * id, p->zslots, sizeof(p->zslots), hipMemcpyHostToDevice, p->stream));
   unsigned long long* bits = &p->ts[3 * id + (int)(p->nstep % 3)].invdt_bits;
   if (p->fp64) {
     ek::Geom<double> g;
@@ -626,17 +663,15 @@ static int admit_upload(ek_pool* p, const ek_params* jp, const double* host, int
     else
       PHIP(p, ek::fp32::inv_dt(jp->dim, (const float*)dst, g, gs, bits, nullptr, p->stream));
   }
-  PHIP(p, hipStreamSynchronize(p->stream));
-  if (p->test_fail_admit == 2)
-    return pfail(p, EK_EHIP, "injected admission failure (after the upload)");
   return EK_OK;
 }
 
-extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host, int layout, int nStepmax,
-                             double tEnd, int* job_id)
+// Checks a job against the pool's kind and reserves a free job slot with its arena range (and,
+// for 3D MHD, its work-area range). The slot is filled but stays unused (out of the work list,
+// status and downloads) until the admission has succeeded; release_job gives everything back.
+// Returns EK_OK and the slot in *id_out.
+static int reserve_job(ek_pool* p, const ek_params* jp, int layout, int nStepmax, double tEnd, int* id_out)
 {
-  if (!p || !jp || !host || !job_id)
-    return pfail(p, EK_EINVAL, "null argument");
   std::string why;
   if (ek::validate_params(jp, why))
     return pfail(p, EK_EINVAL, why);
@@ -665,9 +700,6 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
     arena_free(p, off, need);
     return pfail(p, EK_EFULL, "work buffer full");
   }
-  // The slot is filled but stays unused (out of the work list, status and downloads) until
-  // every step has succeeded; any failure gives the arena range and the slot back, so the
-  // pool is as it was before the call.
   HostJob& h = p->jobs[id];
   h = HostJob{};
   h.p = *jp;
@@ -680,14 +712,36 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
   h.adm_step = p->nstep;
   h.nStepmax = nStepmax;
   h.tEnd = tEnd;
+  h.seq = p->admissions++;
+  *id_out = id;
+  return EK_OK;
+}
+
+// Gives a reserved or admitted job's slot and memory back.
+static void release_job(ek_pool* p, int id)
+{
+  HostJob& h = p->jobs[id];
+  arena_free(p, h.off, h.alloc);
+  if (h.walloc > 0)
+    list_free(p->wfree, h.woff, h.walloc);
+  h = HostJob{};
+}
+
+extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host, int layout, int nStepmax,
+                             double tEnd, int* job_id)
+{
+  if (!p || !jp || !host || !job_id)
+    return pfail(p, EK_EINVAL, "null argument");
+  int id = -1;
+  if (int rc = reserve_job(p, jp, layout, nStepmax, tEnd, &id))
+    return rc;
+  // any failure gives the arena range and the slot back, so the pool is as it was before
   auto rollback = [&](int rc) {
-    p->jobs[id] = HostJob{};
-    arena_free(p, off, need);
-    if (wneed > 0)
-      list_free(p->wfree, woff, wneed);
+    release_job(p, id);
     return rc;
   };
-  if (int rc = admit_upload(p, jp, host, layout, id, off, nelem))
+  HostJob& h = p->jobs[id];
+  if (int rc = admit_upload(p, jp, host, layout, id, h.off, h.nelem))
     return rollback(rc);
   if (int rc = upload_record(p, id))
     return rollback(rc);
@@ -699,6 +753,109 @@ extern "C" int ek_pool_admit(ek_pool* p, const ek_params* jp, const double* host
   if (int rc = rebuild_work(p))
     return rollback(rc); // rebuild_work leaves the previous work list in place on failure
   *job_id = id;
+  return EK_OK;
+}
+
+extern "C" int ek_pool_admit_async(ek_pool* p, const ek_params* jp, const double* host, int nStepmax, double tEnd,
+                                   int* job_id)
+{
+  if (!p || !jp || !host || !job_id)
+    return pfail(p, EK_EINVAL, "null argument");
+  int id = -1;
+  if (int rc = reserve_job(p, jp, EK_LAYOUT_SOA, nStepmax, tEnd, &id))
+    return rc;
+  HostJob& h = p->jobs[id];
+  // once an upload is queued, a failure waits for the upload stream before the range is
+  // given back
+  auto rollback = [&](int rc) {
+    hipStreamSynchronize(p->ustream);
+    release_job(p, id);
+    return rc;
+  };
+  // into the buffer the next pool step reads; activate_pending moves it if the job joins
+  // after an odd number of steps
+  const int par = (int)(p->nstep % 2);
+  char* dst = (char*)p->A[par] + (size_t)h.off * p->elem;
+  hipError_t e;
+  if (p->fp64) {
+    e = hipMemcpyAsync(dst, host, (size_t)h.nelem * 8, hipMemcpyHostToDevice, p->ustream);
+  } else {
+    if (!p->ustage) {
+      const long long n = 1LL << 22;
+      if (hipMalloc(&p->ustage, (size_t)n * sizeof(double)) != hipSuccess) {
+        p->ustage = nullptr;
+        return rollback(pfail(p, EK_ENOMEM, "upload staging allocation failed"));
+      }
+      p->ustage_elems = n;
+    }
+    e = ek::upload_f64_as_f32(host, (float*)dst, h.nelem, p->ustage, p->ustage_elems, p->ustream);
+  }
+  if (e == hipSuccess && !p->jev[id])
+    e = hipEventCreateWithFlags(&p->jev[id], hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipEventRecord(p->jev[id], p->ustream);
+  if (e != hipSuccess)
+    return rollback(pfail(p, EK_EHIP, std::string("asynchronous admission: ") + hipGetErrorString(e)));
+  if (int rc = upload_record(p, id))
+    return rollback(rc);
+  h.used = true;
+  h.pending = true;
+  h.upar = par;
+  h.join_after = p->step_calls + p->test_defer_join;
+  *job_id = id;
+  return EK_OK;
+}
+
+// Lets uploaded jobs join the pool: each pending job whose upload event has completed is
+// moved into the buffer the next step reads (when it was uploaded into the other one), gets
+// its time slots and first invDt, and enters the work lists. mode 0: only jobs already
+// uploaded; 1: waits for the earliest admitted pending job first; 2: waits for all of them.
+static int activate_pending(ek_pool* p, int mode)
+{
+  int first = -1;
+  for (int id = 0; id < p->max_jobs; id++) {
+    const HostJob& h = p->jobs[id];
+    if (h.used && h.pending && (first < 0 || h.seq < p->jobs[first].seq))
+      first = id;
+  }
+  if (first < 0)
+    return EK_OK;
+  if (mode == 1)
+    PHIP(p, hipEventSynchronize(p->jev[first]));
+  else if (mode == 2)
+    PHIP(p, hipStreamSynchronize(p->ustream));
+  bool any = false;
+  for (int id = 0; id < p->max_jobs; id++) {
+    HostJob& h = p->jobs[id];
+    if (!h.used || !h.pending || (mode == 0 && p->step_calls < h.join_after))
+      continue;
+    const hipError_t q = hipEventQuery(p->jev[id]);
+    if (q == hipErrorNotReady)
+      continue;
+    if (q != hipSuccess)
+      return pfail(p, EK_EHIP, std::string("asynchronous admission: ") + hipGetErrorString(q));
+    const int par = (int)(p->nstep % 2);
+    char* dst = (char*)p->A[par] + (size_t)h.off * p->elem;
+    if (par != h.upar)
+      PHIP(p, hipMemcpyAsync(dst, (const char*)p->A[h.upar] + (size_t)h.off * p->elem, (size_t)h.nelem * p->elem,
+                             hipMemcpyDeviceToDevice, p->stream));
+    h.adm_step = p->nstep;
+    if (int rc = init_slots(p, &h.p, id, dst))
+      return rc;
+    h.pending = false;
+    any = true;
+  }
+  if (!any)
+    return EK_OK;
+  PHIP(p, hipStreamSynchronize(p->stream));
+  return rebuild_work(p);
+}
+
+extern "C" int ek_pool_test_defer_join(ek_pool* p, int calls)
+{
+  if (!p || calls < 0)
+    return EK_EINVAL;
+  p->test_defer_join = calls;
   return EK_OK;
 }
 
@@ -714,6 +871,15 @@ extern "C" int ek_pool_step(ek_pool* p, int nsteps)
 {
   if (!p || nsteps < 0)
     return pfail(p, EK_EINVAL, "invalid argument");
+  // jobs whose upload has completed join now; with none stepping, wait for the earliest
+  p->step_calls++;
+  {
+    bool stepping = false;
+    for (const HostJob& h : p->jobs)
+      stepping = stepping || (h.used && !h.pending && !h.finished);
+    if (int rc = activate_pending(p, stepping ? 0 : 1))
+      return rc;
+  }
   int launches = 0;
   PHIP(p, hipEventRecord(p->ev0, p->stream));
   int left = nsteps;
@@ -764,6 +930,9 @@ extern "C" int ek_pool_download(ek_pool* p, int id, double* host, int layout)
 {
   if (!p || !host || id < 0 || id >= p->max_jobs || !p->jobs[id].used)
     return pfail(p, EK_EINVAL, "invalid job");
+  if (p->jobs[id].pending)
+    if (int rc = activate_pending(p, 2))
+      return rc;
   const HostJob& h = p->jobs[id];
   const char* src = (const char*)p->A[job_parity(p, h)] + (size_t)h.off * p->elem;
   if (layout == EK_LAYOUT_SOA) {
@@ -808,6 +977,9 @@ extern "C" int ek_pool_budgets(ek_pool* p, int id, double* sums)
 {
   if (!p || !sums || id < 0 || id >= p->max_jobs || !p->jobs[id].used)
     return pfail(p, EK_EINVAL, "invalid job");
+  if (p->jobs[id].pending)
+    if (int rc = activate_pending(p, 2))
+      return rc;
   const HostJob& h = p->jobs[id];
   const int nv = h.p.nbvar;
   const int nb = (int)std::min<long long>(1024, (h.ncell + 255) / 256);
@@ -840,12 +1012,11 @@ extern "C" int ek_pool_retire(ek_pool* p, int id)
 {
   if (!p || id < 0 || id >= p->max_jobs || !p->jobs[id].used)
     return pfail(p, EK_EINVAL, "invalid job");
-  HostJob& h = p->jobs[id];
-  arena_free(p, h.off, h.alloc);
-  if (h.walloc > 0)
-    list_free(p->wfree, h.woff, h.walloc);
-  h = HostJob{};
-  return rebuild_work(p);
+  const bool pending = p->jobs[id].pending;
+  if (pending)
+    PHIP(p, hipEventSynchronize(p->jev[id])); // its upload still writes the arena range
+  release_job(p, id);
+  return pending ? EK_OK : rebuild_work(p);
 }
 
 static int apply_variant(ek_pool* p, int v)
@@ -873,6 +1044,8 @@ extern "C" int ek_pool_autotune(ek_pool* p, int steps, int* choice, double* ms_o
 {
   if (!p || steps <= 0)
     return pfail(p, EK_EINVAL, "invalid argument");
+  if (int rc = activate_pending(p, 2)) // the snapshot below holds every job's state
+    return rc;
   const int nvar = p->mhd3 ? 1 : (p->kind.dim == 2 ? ek::pool::N2VARIANTS : 4);
   // snapshot of the current buffer and the time slots, restored after every trial
   long long hi = 0;

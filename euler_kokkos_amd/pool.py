@@ -26,7 +26,8 @@ EK_EFULL = 4
 POOL_EXPORTS = ("ek_pool_create", "ek_pool_destroy", "ek_pool_last_error", "ek_pool_job_bytes",
                 "ek_pool_free_bytes", "ek_pool_active", "ek_pool_admit", "ek_pool_step",
                 "ek_pool_status", "ek_pool_download", "ek_pool_budgets", "ek_pool_retire",
-                "ek_pool_autotune", "ek_pool_set_tile", "ek_pool_last_step_time", "ek_pool_test_fail_admit")
+                "ek_pool_autotune", "ek_pool_set_tile", "ek_pool_last_step_time", "ek_pool_test_fail_admit",
+                "ek_pool_admit_async", "ek_pool_test_defer_join")
 
 _BOUND = False
 
@@ -48,6 +49,7 @@ def _lib() -> C.CDLL:
         L.ek_pool_free_bytes.restype = C.c_longlong
         L.ek_pool_active.argtypes = [V]
         L.ek_pool_admit.argtypes = [V, P(EkParams), D, C.c_int, C.c_int, C.c_double, P(C.c_int)]
+        L.ek_pool_admit_async.argtypes = [V, P(EkParams), D, C.c_int, C.c_double, P(C.c_int)]
         L.ek_pool_step.argtypes = [V, C.c_int]
         L.ek_pool_status.argtypes = [V, C.c_int, D, P(C.c_int), P(C.c_int)]
         L.ek_pool_download.argtypes = [V, C.c_int, D, C.c_int]
@@ -57,6 +59,7 @@ def _lib() -> C.CDLL:
         L.ek_pool_set_tile.argtypes = [V, C.c_int]
         L.ek_pool_last_step_time.argtypes = [V, D, P(C.c_int)]
         L.ek_pool_test_fail_admit.argtypes = [V, C.c_int]
+        L.ek_pool_test_defer_join.argtypes = [V, C.c_int]
         _BOUND = True
     return L
 
@@ -75,6 +78,7 @@ class Pool:
         self.h = h
         self.kind = kind
         self.params = {}
+        self._inputs = {}  # host arrays of asynchronously admitted jobs, held until retire
 
     def _check(self, rc, what):
         if rc == EK_EFULL:
@@ -103,11 +107,23 @@ class Pool:
         return int(self._L.ek_pool_active(self.h))
 
     def admit(self, p: EkParams, U: np.ndarray, nStepmax: int, tEnd: float = 1e30,
-              layout: int = EK_LAYOUT_SOA) -> int:
+              layout: int = EK_LAYOUT_SOA, wait: bool = True) -> int:
+        """Admit a job. With wait=False (SoA layout only) the upload runs on the pool's upload
+        stream while later steps advance the other jobs, and the job joins the first step after
+        it has completed; the host array is held until the job is retired (page-locked memory,
+        e.g. native.pinned_copy, lets the copy overlap the steps)."""
         a = np.ascontiguousarray(U, dtype=np.float64)
         jid = C.c_int(-1)
-        self._check(self._L.ek_pool_admit(self.h, C.byref(p), a.ctypes.data_as(C.POINTER(C.c_double)),
-                                          layout, int(nStepmax), float(tEnd), C.byref(jid)), "ek_pool_admit")
+        ptr = a.ctypes.data_as(C.POINTER(C.c_double))
+        if wait:
+            self._check(self._L.ek_pool_admit(self.h, C.byref(p), ptr, layout, int(nStepmax), float(tEnd),
+                                              C.byref(jid)), "ek_pool_admit")
+        else:
+            if layout != EK_LAYOUT_SOA:
+                raise ValueError("asynchronous admission takes the SoA layout only")
+            self._check(self._L.ek_pool_admit_async(self.h, C.byref(p), ptr, int(nStepmax), float(tEnd),
+                                                    C.byref(jid)), "ek_pool_admit_async")
+            self._inputs[jid.value] = a
         self.params[jid.value] = p
         return jid.value
 
@@ -137,6 +153,7 @@ class Pool:
     def retire(self, jid: int) -> None:
         self._check(self._L.ek_pool_retire(self.h, jid), "ek_pool_retire")
         self.params.pop(jid, None)
+        self._inputs.pop(jid, None)
 
     def autotune(self, steps: int = 3):
         ch = C.c_int(-1)
@@ -158,6 +175,11 @@ class Pool:
         """Make every following admission fail at `stage` (1 before the upload, 2 after it, 3
         after the job's device record is written; 0 turns it off) -- for rollback tests."""
         self._check(self._L.ek_pool_test_fail_admit(self.h, int(stage)), "ek_pool_test_fail_admit")
+
+    def test_defer_join(self, calls: int) -> None:
+        """Asynchronously admitted jobs join no earlier than the `calls`-th step() call after
+        their admission while other jobs are stepping (0 turns it off) -- for tests."""
+        self._check(self._L.ek_pool_test_defer_join(self.h, int(calls)), "ek_pool_test_defer_join")
 
 
 # ------------------------------------------------------------------ seeded generator

@@ -3,7 +3,8 @@ their HBM footprint and advances every admitted job together (continuous batchin
 
 Jobs arrive on a seeded Poisson process. Each job kind has its own pool and its own loop,
 the loops running side by side: at every turn a loop admits, in arrival order, each job of
-its kind that has arrived and fits in the pool's memory; then it advances all admitted jobs
+its kind that has arrived and fits in the pool's memory (its upload runs on the pool's upload
+stream beside the steps, and it joins the first step after the upload); then it advances all admitted jobs
 of the pool together (one persistent launch per step for a hydro pool, the five 3D MHD
 kernels each over all jobs for an MHD pool, replayed from a captured graph); finished jobs
 are collected and retired, which frees memory for the queue. Time-to-solution is measured
@@ -150,7 +151,9 @@ class Server:
             while pending and pending[0].arrival <= now:
                 j = pending[0]
                 try:
-                    jid = pool.admit(j.params, j.U0, j.steps)
+                    # the upload runs beside the steps of the jobs already admitted; the job
+                    # joins the first step after it has completed
+                    jid = pool.admit(j.params, j.U0, j.steps, wait=False)
                 except PoolFull:
                     break  # first come, first served: wait for memory
                 except native.EkError as e:  # rejected (e.g. invalid parameters): fail this job only

@@ -53,6 +53,18 @@ int ek_pool_active(const ek_pool* pool);
 int ek_pool_admit(ek_pool* pool, const ek_params* job, const double* host_state, int layout, int nStepmax,
                   double tEnd, int* job_id);
 
+/* ek_pool_admit without waiting for the upload (layout EK_LAYOUT_SOA): the copy of the
+ * initial state (and its rounding in an FP32 pool) runs on the pool's upload stream while
+ * ek_pool_step keeps advancing the other jobs, and the job joins the first ek_pool_step that
+ * starts after the upload has completed (t = 0, it = 0 until then; an ek_pool_step with no
+ * other job to advance waits for the earliest such upload). host_state must stay valid and
+ * unchanged until the job is retired; page-locked memory (ek_host_alloc) lets the copy run
+ * beside the steps. ek_pool_download / ek_pool_budgets / ek_pool_autotune wait for pending
+ * uploads; ek_pool_retire of a job still uploading waits for its upload. (No reference
+ * counterpart.) */
+int ek_pool_admit_async(ek_pool* pool, const ek_params* job, const double* host_state, int nStepmax, double tEnd,
+                        int* job_id);
+
 /* Advance every unfinished job by nsteps steps (a job stops at its own tEnd / nStepmax).
  * Synchronizes and refreshes the jobs' t / it / finished. */
 int ek_pool_step(ek_pool* pool, int nsteps);
@@ -84,6 +96,11 @@ int ek_pool_last_step_time(const ek_pool* pool, double* ms, int* launches);
  * after its device record is written; stage 0 turns it off. A failed admission leaves the
  * pool as it was (arena, job slots, work list). */
 int ek_pool_test_fail_admit(ek_pool* pool, int stage);
+
+/* Test hook for asynchronous admission: a job admitted by a following ek_pool_admit_async
+ * joins no earlier than the `calls`-th ek_pool_step call after its admission while other jobs
+ * are stepping (so that tests can make it join after an odd number of steps); 0 turns it off. */
+int ek_pool_test_defer_join(ek_pool* pool, int calls);
 
 #ifdef __cplusplus
 }

@@ -204,3 +204,61 @@ def test_pool_mhd3d_matches_single_grid(precision):
         assert P.free_bytes() == free0 and P.active() == 0
     finally:
         P.close()
+
+
+def jobs3d(precision):
+    out = []
+    for nx, ny, nz, bc in [(32, 24, 20, (3,) * 6), (40, 32, 48, (1, 2, 3, 3, 1, 1)), (24, 40, 28, (2,) * 6)]:
+        p = make_params("Hydro_Muscl_3D", nx, ny, nz, gamma0="1.4", cfl="0.5", slope_type="2",
+                        riemann="hllc", bc=bc, precision=precision)
+        out.append((p, problems.blast(p, radius="0.2", density_out="1", pressure_out="0.1")))
+    return out
+
+
+@pytest.mark.parametrize("kind,precision", [("hydro2d", native.EK_FP64), ("hydro3d", native.EK_FP32),
+                                            ("hydro3d", native.EK_FP64), ("mhd3d", native.EK_FP32)])
+def test_async_admission_matches_single_grid(kind, precision):
+    # jobs admitted without waiting for their upload join while another job steps: one after
+    # an odd number of steps (its state is moved to the other arena buffer), one after an even
+    # number, one at the next step; each ends bitwise at its single-grid result. A job retired
+    # while still uploading, and one downloaded before it joined, leave the pool consistent.
+    jobs = {"hydro2d": jobs2d, "hydro3d": jobs3d, "mhd3d": jobs_mhd3d}[kind](precision)[:3]
+    steps = [12, 9, 7]
+    ref = [single_run(p, U, s) for (p, U), s in zip(jobs, steps)]
+    P = pool.Pool(jobs[0][0], 512 << 20, 8)
+    try:
+        free0 = P.free_bytes()
+        inputs = [native.pinned_copy(U) for _, U in jobs]
+        ids = [P.admit(jobs[0][0], inputs[0], steps[0], wait=False)]
+        P.step(2)                   # nothing else to step: the first step waits for its upload
+        assert P.status(ids[0])[1] == 2
+        P.test_defer_join(2)        # joins at the second step() call: after one more step
+        ids.append(P.admit(jobs[1][0], inputs[1], steps[1], wait=False))
+        P.step(1)
+        assert P.status(ids[1])[1] == 0
+        P.test_defer_join(3)        # after two more steps
+        ids.append(P.admit(jobs[2][0], inputs[2], steps[2], wait=False))
+        P.step(1)
+        P.step(1)
+        # a job that is retired before it joins, and one downloaded before it joins
+        P.test_defer_join(0)
+        gone = P.admit(jobs[0][0], inputs[0], 5, wait=False)
+        P.retire(gone)
+        early = P.admit(jobs[1][0], inputs[1], 3, wait=False)
+        D = P.download(early)       # waits for the upload: the initial state as the pool holds it
+        E = single_run(jobs[1][0], jobs[1][1], 0)[0]
+        assert np.array_equal(D, E)
+        P.retire(early)
+        P.step(30)
+        for k, jid in enumerate(ids):
+            t, it, fin = P.status(jid)
+            G = P.download(jid)
+            print(f"case=pool async {kind} prec={precision} job={k}: it={it} "
+                  f"max|diff|={np.abs(G - ref[k][0]).max():.3e}", flush=True)
+            assert fin and it == steps[k] == ref[k][2] and t == ref[k][1]
+            assert np.array_equal(G, ref[k][0])
+        for jid in ids:
+            P.retire(jid)
+        assert P.free_bytes() == free0 and P.active() == 0
+    finally:
+        P.close()

@@ -45,7 +45,7 @@ class _FakePool:
         self.next = 0
         self.async_failures = async_failures
 
-    def admit(self, params, U, steps):
+    def admit(self, params, U, steps, wait=True):
         if params == "reject":
             raise native.EkError("ek_create failed (2): invalid parameters")
         jid, self.next = self.next, self.next + 1
