@@ -102,14 +102,24 @@ def cpu_baseline(n, seconds):
 
 
 _HOST = None
+_HOST_CPUS = None
+
+
+def unpin():
+    """Gives this thread back the CPUs it started with. The OpenMP runtime binds the thread
+    that runs its first parallel region (a CPU baseline) to one place (OMP_PROC_BIND=close),
+    and threads started afterwards -- the serving loops -- would inherit that single CPU."""
+    if _HOST_CPUS:
+        os.sched_setaffinity(0, _HOST_CPUS)
 
 
 def host_info():
     """CPU model and OpenMP placement of the host the CPU baseline runs on (taken once, at the
     first call, before any OpenMP region binds the calling thread to one place)."""
-    global _HOST
+    global _HOST, _HOST_CPUS
     if _HOST is not None:
         return dict(_HOST)
+    _HOST_CPUS = set(os.sched_getaffinity(0))
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -339,6 +349,7 @@ def serving_c5(n_jobs, rate, seed, steps):
                        f"FP32 state, FP64 dt",
            "kinds": ["mhd_blast", "hydro_kh"]}
     jobs = serving.poisson_stream(n_jobs, rate, seed, steps=steps, precision=native.EK_FP32)
+    res["host_cpus_serving"] = len(os.sched_getaffinity(0))
     progress(f"serving: {n_jobs} job states built")
     for name, burst in (("stream", False), ("burst", True)):
         # stream: the Poisson arrival times (latency at the offered load); burst: every job
@@ -505,14 +516,17 @@ def main() -> int:
         progress("headline FP32 done")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, a.cpu_seconds)
+        unpin()
         progress("CPU baseline done")
     else:
         out["cpu_baseline"] = None
     if not a.no_mhd:
         out["mhd2d"] = mhd_c3(a.mhd_steps, not a.no_cpu, a.cpu_seconds / 2)
+        unpin()
         progress("2D MHD done")
     if not a.no_mhd3d:
         out["mhd3d"] = mhd3d_c5(a.mhd3d_n, a.mhd3d_steps, not a.no_cpu, a.cpu_seconds / 2)
+        unpin()
         progress("3D MHD done")
     if not a.no_batch:
         out["batch2d"] = batch_c4(a.batch_runs, a.batch_steps, a.batch_seed)
