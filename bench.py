@@ -446,7 +446,7 @@ def main() -> int:
     ap.add_argument("--serve-seed", type=int, default=77)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}): this build has no multi-GPU "
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}): the library has no multi-GPU "
               "path (a grid would shard into one subdomain per GPU with RCCL halo exchanges, "
               "and batched jobs by job).", file=sys.stderr)
         return 2
