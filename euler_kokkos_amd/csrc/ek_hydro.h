@@ -276,7 +276,9 @@ __device__ __forceinline__ void step_end_bookkeeping(const StepCtl& c, const Tim
 }
 
 // CFL term of one cell of the new state. The FP64 state uses the reference's expression
-// as is; an FP32 state is widened so that the dt reduction always runs in FP64.
+// as is; an FP32 state is widened so that the dt reduction always runs in FP64, with one
+// FP64 division (1/rho) where the reference's expression has five (FP32 3D step 0.553 ->
+// 0.538 ms at 256^3; the divisions by the cell size stay: a hoisted 1/dx costs registers).
 template <class T, int DIM>
 __device__ __forceinline__ double cfl_term(const Gas<T>& gas, const T* u, const Geom<T>& geo)
 {
@@ -284,18 +286,22 @@ __device__ __forceinline__ double cfl_term(const Gas<T>& gas, const T* u, const 
     return (double)cell_inv_dt<T, DIM>(gas, u, geo.tdx, geo.tdy, geo.tdz, geo.idx, geo.idy, geo.idz,
                                        geo.pow2 != 0);
   } else {
-    constexpr int NV = DIM == 3 ? 5 : 4;
-    Gas<double> g;
-    g.gamma0 = gas.gamma0;
-    g.gm1 = gas.gm1;
-    g.smallr = gas.smallr;
-    g.smallp = gas.smallp;
-    double ud[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++)
-      ud[v] = (double)u[v];
-    return cell_inv_dt<double, DIM>(g, ud, geo.dx, geo.dy, geo.dz, 1.0 / geo.dx, 1.0 / geo.dy,
-                                    1.0 / geo.dz, false);
+    const double d = fmax((double)u[ID], (double)gas.smallr);
+    const double id = 1.0 / d;
+    const double ux = (double)u[IU] * id, uy = (double)u[IV] * id;
+    double ek = ux * ux + uy * uy;
+    double uz = 0.0;
+    if constexpr (DIM == 3) {
+      uz = (double)u[IW] * id;
+      ek += uz * uz;
+    }
+    const double e = (double)u[IP] * id - 0.5 * ek;
+    const double p = fmax((double)gas.gm1 * d * e, d * (double)gas.smallp);
+    const double c = sqrt((double)gas.gamma0 * p * id);
+    double s = (c + fabs(ux)) / geo.dx + (c + fabs(uy)) / geo.dy;
+    if constexpr (DIM == 3)
+      s += (c + fabs(uz)) / geo.dz;
+    return s;
   }
 }
 

@@ -588,24 +588,7 @@ __device__ __forceinline__ double cfl3(const Gas<T>& gas, const T* u, T mx, T my
       return vx * geo.idx + vy * geo.idy + vz * geo.idz;
     return vx / geo.dx + vy / geo.dy + vz / geo.dz;
   } else {
-    Gas<double> g;
-    g.gamma0 = gas.gamma0;
-    g.gm1 = gas.gm1;
-    g.smallr = gas.smallr;
-    g.smallp = gas.smallp;
-    double ud[NV], q[NV];
-#pragma unroll
-    for (int v = 0; v < NV; v++)
-      ud[v] = (double)u[v];
-    constoprim<double>(g, ud, (double)mx, (double)my, (double)mz, q);
-    const double d = q[ID], p = q[IP], a = q[IA], b = q[IB], c = q[IC];
-    const double b2 = a * a + b * b + c * c;
-    const double c2 = g.gamma0 * p / d;
-    const double d2 = 0.5 * (b2 / d + c2);
-    const double vx = sqrt(d2 + sqrt(d2 * d2 - c2 * a * a / d)) + fabs(q[IU]);
-    const double vy = sqrt(d2 + sqrt(d2 * d2 - c2 * b * b / d)) + fabs(q[IV]);
-    const double vz = sqrt(d2 + sqrt(d2 * d2 - c2 * c * c / d)) + fabs(q[IW]);
-    return vx / geo.dx + vy / geo.dy + vz / geo.dz;
+    return mhd_cfl_widened<3>(gas, u, (double)mx, (double)my, (double)mz, geo.dx, geo.dy, geo.dz);
   }
 }
 
