@@ -152,3 +152,25 @@ def test_mhd3d_fp32_l1():
     l1 = np.abs(out[1][ID] - out[0][ID]).mean() / np.abs(out[0][ID]).mean()
     print(f"case=mhd3d FP32 vs FP64 48^3: rel L1(rho)={l1:.3e}", flush=True)
     assert np.isfinite(out[1]).all() and l1 <= 1e-4
+
+
+@pytest.mark.parametrize("bc", [(1, 2, 3, 3, 2, 1), (2, 2, 1, 1, 3, 3)])
+def test_mhd3d_fp32_ragged_mixed_bc_l1(bc):
+    # the FP32 pipeline on a ragged grid with reflecting / outflow / periodic sides: relative
+    # L1 of every variable against the FP64 pipeline (bitwise the oracle's) within 1e-4
+    out = []
+    for prec in (native.EK_FP64, native.EK_FP32):
+        ekp = m3(29, 21, 17, bc=bc, precision=prec)
+        ctx = native.Context(ekp)
+        try:
+            ctx.upload(problems.mhd_blast3d(ekp, B0=0.5), native.EK_LAYOUT_SOA)
+            t, it, _ = ctx.run(20, 20, 1e30)
+            out.append(ctx.download(native.EK_LAYOUT_SOA))
+        finally:
+            ctx.close()
+        assert it == 20
+    assert np.isfinite(out[1]).all()
+    for v in range(8):
+        l1 = np.abs(out[1][v] - out[0][v]).mean() / np.abs(out[0][v]).mean()
+        print(f"case=mhd3d FP32 bc={bc} var={v}: rel L1 {l1:.3e}", flush=True)
+        assert l1 <= 1e-4

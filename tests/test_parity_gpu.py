@@ -234,3 +234,30 @@ def test_fp32_bulk_l1_vs_fp64(oracle):
     l1_e = np.abs(out[1][1] - out[0][1]).mean() / np.abs(out[0][1]).mean()
     print(f"case=fp32 vs fp64 48^3: L1 rho={l1_rho:.3e} L1 E(rel)={l1_e:.3e}", flush=True)
     assert l1_rho <= 1e-4 and l1_e <= 1e-4
+
+
+@pytest.mark.parametrize("bc", [(1, 2, 3, 3, 2, 1), (2, 2, 1, 1, 3, 3), (1,) * 6])
+def test_fp32_bulk_ragged_mixed_bc_l1(oracle, bc):
+    # FP32 bulk on a grid that is not a multiple of the tile or of the z chunk, with
+    # reflecting / outflow / periodic sides: relative L1 of every variable against the FP64
+    # step (itself bitwise the oracle's on these boundaries) within 1e-4
+    kw = dict(riemann="hllc", slope_type="2", cfl="0.6", bc=bc, xmax="1.3")
+    ekp64 = make_params("Hydro_Muscl_3D", 37, 21, 19, **kw)
+    ekp32 = make_params("Hydro_Muscl_3D", 37, 21, 19, precision=native.EK_FP32, **kw)
+    U0 = oracle.blast(oracle.from_ek(ekp64), radius=0.25, center_x=0.3, center_y=0.6, center_z=0.4,
+                      density_in=2.0, density_out=1.0, pressure_in=5.0, pressure_out=0.2)
+    out = []
+    for ekp in (ekp64, ekp32):
+        ctx = native.Context(ekp)
+        try:
+            ctx.upload(U0)
+            t, it, _ = ctx.run(25, 25, 1e30)
+            out.append(ctx.download(native.EK_LAYOUT_SOA))
+        finally:
+            ctx.close()
+        assert it == 25
+    assert np.isfinite(out[1]).all()
+    for v in range(5):
+        l1 = np.abs(out[1][v] - out[0][v]).mean() / np.abs(out[0][v]).mean()
+        print(f"case=hydro3d FP32 bc={bc} var={v}: rel L1 {l1:.3e}", flush=True)
+        assert l1 <= 1e-4
